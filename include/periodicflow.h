@@ -1,0 +1,156 @@
+/*
+ * periodicflow C ABI: the Fourier-Galerkin right-hand side and time stepping
+ * of periodicflow on one MI355X (gfx950), as hand-written HIP kernels.
+ *
+ * The reference is a pure-Python package with no native interface; these
+ * entry points replace the array work behind its Python API:
+ *
+ *   pf_plan_create           FourierSpace.__init__  src/periodicflow/basis.py:34-92
+ *                            SpaceSolver.__init__   src/periodicflow/space_solver.py:107-150
+ *                            TimeIntegrator.__init__ src/periodicflow/time_integrator.py:17-29
+ *   pf_set_linear_operator   Solver.set_linear_operator src/periodicflow/simulation.py:182-184
+ *                            TimeIntegrator.update_linear_operator time_integrator.py:31-33
+ *   pf_set_correlation       Solver.set_correlation_function simulation.py:186-187
+ *   pf_forward / pf_backward SpaceSolver.forward / backward space_solver.py:215-225
+ *   pf_project               math.leray_projection (as called by Solver.initialize,
+ *                            simulation.py:170-177)
+ *   pf_rhs_nonlinear         SpaceSolver.compute_rhs_nonlinear space_solver.py:286-300
+ *   pf_step                  the loop body of Solver.solve simulation.py:205-213:
+ *                            u0 = u_hat; random_fields(); per stage compute_rhs(a)
+ *                            (space_solver.py:302-310) then the integrator's stepping
+ *   pf_compute_rhs           SpaceSolver.compute_rhs space_solver.py:302-310
+ *   pf_stage_update          <integrator>.stepping time_integrator.py:51-318
+ *   pf_draw_noise            SpaceSolver.random_fields space_solver.py:227-238
+ *   pf_diagnostics, pf_spectrum, pf_accumulate_sk
+ *                            FP64 reductions; no counterpart in the reference
+ *
+ * Conventions: every array argument is a device pointer in the plan's
+ * precision (float / double; complex = interleaved pairs), C order, one
+ * member after another.  u_hat is (batch, dim, N0, [N1,] N_last/2+1), the
+ * physical velocity (batch, dim, N0, [N1,] N_last).  All work is enqueued
+ * on the given stream (0 = null stream).  Return value 0 on success or a
+ * negative PF_E* code; pf_last_error() describes the last failure of the
+ * calling thread.  A plan is not thread safe.
+ */
+#ifndef PERIODICFLOW_H
+#define PERIODICFLOW_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PF_OK 0
+#define PF_EINVAL (-1)
+#define PF_ENOMEM (-2)
+#define PF_EHIP (-3)
+#define PF_EUNSUPPORTED (-4)
+
+enum { PF_FP32 = 0, PF_FP64 = 1 };
+enum { PF_DEALIAS_PAD32 = 0, PF_DEALIAS_TRUNC23 = 1 };
+enum { PF_EXPLICIT_PC = 0, PF_IMPLICIT_PC = 1, PF_RK2 = 2, PF_RK3 = 3, PF_RK4 = 4 };
+enum { PF_NOISE_THERMAL = 0, PF_NOISE_CORRELATED = 1 };
+enum { PF_FORCING_NONE = 0, PF_FORCING_CONSTANT_POWER = 1 };
+
+typedef struct pf_desc {
+  int32_t dim;          /* 2 or 3 */
+  int32_t n[3];         /* grid points per axis (even; 2^a 3^b) */
+  double length[3];     /* box lengths; wavenumbers are scaled by 2 pi / length */
+  int32_t precision;    /* PF_FP32 / PF_FP64 */
+  int32_t dealias;      /* PF_DEALIAS_PAD32 ('3/2') / PF_DEALIAS_TRUNC23 ('2/3') */
+  int32_t mask_nyquist; /* zero the Nyquist planes of the nonlinear term */
+  int32_t integrator;   /* PF_EXPLICIT_PC .. PF_RK4 */
+  int32_t nonlinear;    /* 0: Stokes, 1: Navier-Stokes */
+  int32_t noise_type;   /* PF_NOISE_THERMAL / PF_NOISE_CORRELATED */
+  int32_t forcing;      /* PF_FORCING_NONE / PF_FORCING_CONSTANT_POWER */
+  int32_t batch;        /* members advanced together (same shape and parameters) */
+  double viscosity;
+  double dt;
+  double noise_mag;
+  double forcing_power;  /* P of f = P / (2 E_band) u_hat */
+  double forcing_kmin;   /* band kmin <= |k| <= kmax */
+  double forcing_kmax;
+} pf_desc;
+
+typedef struct pf_plan pf_plan;
+
+typedef struct pf_plan_info {
+  int64_t n_fourier;       /* complex modes per component per member */
+  int64_t n_physical;      /* real points per component per member */
+  int32_t padded[3];       /* transform lengths of the padded grid */
+  int32_t noise_components;/* d*d (thermal) or d (correlated) */
+  int64_t workspace_bytes; /* device memory owned by the plan */
+  int64_t state_bytes;     /* bytes of one u_hat (all members) */
+} pf_plan_info;
+
+int pf_plan_create(const pf_desc* desc, pf_plan** plan);
+int pf_plan_destroy(pf_plan* plan);
+int pf_plan_get_info(const pf_plan* plan, pf_plan_info* info);
+
+/* Per-member RNG seeds (host array of `batch` values) for on-device noise. */
+int pf_set_seeds(pf_plan* plan, const uint64_t* seeds);
+/* Step counter that keys the on-device noise (starts at 0). */
+int pf_set_step_counter(pf_plan* plan, int64_t step);
+int64_t pf_get_step_counter(const pf_plan* plan);
+
+/* L: device array (N0, [N1,] nh) of reals, shared by all members; NULL
+ * restores -nu |k|^2.  Copied into the plan. */
+int pf_set_linear_operator(pf_plan* plan, const void* L, void* stream);
+/* C: device array (N0, [N1,] nh) of complex values.  Copied into the plan. */
+int pf_set_correlation(pf_plan* plan, const void* C, void* stream);
+
+/* Unpadded transforms of `ncomp` components per member.
+ * forward: normalised by 1/prod(N); backward: unnormalised. */
+int pf_forward(pf_plan* plan, const void* u, void* u_hat, int32_t ncomp, void* stream);
+int pf_backward(pf_plan* plan, const void* u_hat, void* u, int32_t ncomp, void* stream);
+
+/* In-place Leray projection of u_hat (all members). */
+int pf_project(pf_plan* plan, void* u_hat, void* stream);
+/* Mask the k = 0 mode of u_hat in place (Solver.initialize mask_zero_mode). */
+int pf_mask_zero_mode(pf_plan* plan, void* u_hat, void* stream);
+
+/* Nonlinear term P(F(u x omega)) with the plan's dealiasing (Nyquist mask
+ * applied when mask_nyquist is set).  rhs may not alias u_hat. */
+int pf_rhs_nonlinear(pf_plan* plan, const void* u_hat, void* rhs, void* stream);
+
+/* Advance u_hat by n_steps time steps in place.
+ * noise == NULL: noise drawn on the device (Philox4x32-10 keyed by the
+ *   member seed, the step counter, the mode and the component).
+ * noise != NULL: raw standard-normal draws, n_steps blocks of
+ *   (batch, 2, noise_components, nk) complex values (w1 then w2), the
+ *   quantities np.random.randn fills in random_fields before scaling.
+ * The step counter advances by n_steps. */
+int pf_step(pf_plan* plan, void* u_hat, int64_t n_steps, const void* noise, void* stream);
+
+/* compute_rhs(a) of one stage (space_solver.py:302-310): writes the
+ * projected (and Nyquist-masked) nonlinear term, L u_hat, the forcing and
+ * the projected noise built from w1 + a w2 into four arrays shaped like
+ * u_hat.  noise: raw draws (batch, 2, noise_components, nk) of the current
+ * step, or NULL for the on-device stream at the current step counter. */
+int pf_compute_rhs(pf_plan* plan, void* u_hat, double a, const void* noise, void* rhs_nonlinear,
+                   void* rhs_linear, void* forcing, void* noise_out, void* stream);
+/* The plan integrator's stepping(unew, u0, stage, ...) from given terms
+ * (time_integrator.py:51-318).  implicit_pc overwrites rhs_linear with
+ * L u0 at its last stage, as the reference does (:116-117). */
+int pf_stage_update(pf_plan* plan, int32_t stage, void* unew, const void* u0, const void* rhs_nonlinear,
+                    void* rhs_linear, const void* forcing, const void* noise, void* stream);
+/* random_fields (space_solver.py:227-238) on the device: raw unscaled
+ * draws (batch, 2, noise_components, nk) for the current step counter. */
+int pf_draw_noise(pf_plan* plan, void* out, void* stream);
+
+/* FP64 diagnostics.  out: device doubles (batch, 2) = energy 1/2<|u|^2>,
+ * enstrophy 1/2<|omega|^2>.  Overwritten. */
+int pf_diagnostics(pf_plan* plan, const void* u_hat, double* out, void* stream);
+/* Shell spectrum E(k), shells round(|k|): out device doubles (batch, nshell), overwritten. */
+int pf_spectrum(pf_plan* plan, const void* u_hat, double* out, int32_t nshell, void* stream);
+/* acc (device doubles, nk) += sum over members and components |u_hat|^2. */
+int pf_accumulate_sk(pf_plan* plan, const void* u_hat, double* acc, void* stream);
+
+const char* pf_last_error(void);
+const char* pf_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

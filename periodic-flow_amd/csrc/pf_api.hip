@@ -1,0 +1,754 @@
+// C ABI: plans, workspace, and the per-stage launch sequence.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/periodicflow.h"
+#include "pf_launch.h"
+
+using namespace pf;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  return fail(PF_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define PF_HIP(call)                                   \
+  do {                                                 \
+    hipError_t e_ = (call);                            \
+    if (e_ != hipSuccess) return hip_fail(e_, #call);  \
+  } while (0)
+
+#define PF_TRY(call)              \
+  do {                            \
+    int rc_ = (call);             \
+    if (rc_) return launch_fail(rc_, #call); \
+  } while (0)
+
+int launch_fail(int rc, const char* what) {
+  if (rc == -4) return fail(PF_EUNSUPPORTED, std::string("unsupported transform length in ") + what);
+  if (rc == -3) return fail(PF_EHIP, std::string("kernel launch failed: ") + what + ": " +
+                                         hipGetErrorString(hipGetLastError()));
+  return fail(PF_EINVAL, std::string("invalid launch: ") + what);
+}
+
+}  // namespace
+
+struct pf_plan {
+  pf_desc d{};
+  int dim = 0;
+  int N[3] = {1, 1, 1};
+  int M[3] = {1, 1, 1};
+  int nh = 0;
+  long long nk = 0, nphys = 0;
+  int B = 1;
+  bool fp64 = false;
+  size_t rsz = 4, csz = 8;
+  int nc = 0;
+  bool direct23 = false;
+  double ks[3] = {1, 1, 1};
+  void* u0 = nullptr;
+  void* w1 = nullptr;
+  void* w2 = nullptr;
+  size_t w1_bytes = 0, w2_bytes = 0;
+  void* lin_op = nullptr;
+  void* corr = nullptr;
+  unsigned long long* seeds = nullptr;
+  double* fcoef = nullptr;
+  std::map<int, void*> tw;
+  long long step = 0;
+  size_t bytes = 0;
+  std::vector<void*> owned;
+};
+
+namespace {
+
+int dev_alloc(pf_plan* p, void** ptr, size_t bytes) {
+  if (bytes == 0) bytes = 256;
+  hipError_t e = hipMalloc(ptr, bytes);
+  if (e != hipSuccess) {
+    *ptr = nullptr;
+    return fail(PF_ENOMEM, std::string("hipMalloc of ") + std::to_string(bytes) + " bytes: " + hipGetErrorString(e));
+  }
+  p->owned.push_back(*ptr);
+  p->bytes += bytes;
+  return 0;
+}
+
+void release(pf_plan* p) {
+  for (void* q : p->owned) (void)hipFree(q);
+  p->owned.clear();
+}
+
+bool supported_length(int L) {
+  switch (L) {
+#define PF_CASE(n) case n: return true;
+    PF_LENGTHS(PF_CASE)
+#undef PF_CASE
+    default: return false;
+  }
+}
+
+int make_twiddles(pf_plan* p, int L) {
+  if (p->tw.count(L)) return 0;
+  void* dptr = nullptr;
+  int rc = dev_alloc(p, &dptr, (size_t)L * p->csz);
+  if (rc) return rc;
+  const long double twopi = 6.283185307179586476925286766559005768L;
+  if (p->fp64) {
+    std::vector<double> h(2 * L);
+    for (int m = 0; m < L; ++m) {
+      const long double ang = twopi * (long double)m / (long double)L;
+      h[2 * m] = (double)cosl(ang);
+      h[2 * m + 1] = (double)-sinl(ang);
+    }
+    PF_HIP(hipMemcpy(dptr, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> h(2 * L);
+    for (int m = 0; m < L; ++m) {
+      const long double ang = twopi * (long double)m / (long double)L;
+      h[2 * m] = (float)cosl(ang);
+      h[2 * m + 1] = (float)-sinl(ang);
+    }
+    PF_HIP(hipMemcpy(dptr, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+  }
+  p->tw[L] = dptr;
+  return 0;
+}
+
+// geometry block shared by the point-wise kernels
+StageArgs geometry(const pf_plan* p) {
+  StageArgs g{};
+  g.nk = p->nk;
+  g.n_members = p->B;
+  g.nh = p->nh;
+  g.N0 = p->N[0];
+  g.N1 = p->N[1];
+  g.N2 = p->dim == 3 ? p->N[2] : p->N[1];
+  g.ks0 = p->ks[0];
+  g.ks1 = p->ks[1];
+  g.ks2 = p->dim == 3 ? p->ks[2] : 0.0;
+  g.nu = p->d.viscosity;
+  return g;
+}
+
+template <typename T> int col(const pf_plan* p, int L, const ColArgs& a, hipStream_t s) {
+  return launch_col<T>(L, a, p->tw.at(L), s);
+}
+template <typename T> int row(const pf_plan* p, int L, int mode, const RowArgs& a, hipStream_t s) {
+  return launch_row<T>(L, mode, a, p->tw.at(L), s);
+}
+
+ColArgs col_args() {
+  ColArgs a{};
+  a.n_outer = 1;
+  a.n_members = 1;
+  a.n_fields = 1;
+  a.scale = 1.0;
+  a.mode = COL_PLAIN;
+  return a;
+}
+
+// Nonlinear term (unprojected) of u_hat into the plan's R buffer (w1).
+template <typename T> int nonlinear_unprojected(pf_plan* p, const void* uhat, hipStream_t s) {
+  const int B = p->B;
+  const long long nh = p->nh;
+  if (p->dim == 3) {
+    const int N0 = p->N[0], N1 = p->N[1], N2 = p->N[2];
+    const int M0 = p->M[0], M1 = p->M[1], M2 = p->M[2];
+    const long long A_f = (long long)M0 * N1 * nh, B_f = (long long)M0 * M1 * nh;
+    const long long D_f = (long long)M0 * N1 * nh;
+    // P1: axis-0 inverse with curl, u_hat -> A (6 fields)
+    ColArgs a = col_args();
+    a.in = uhat; a.out = p->w1;
+    a.rows_in = N0; a.rows_out = M0; a.cols = N1 * (int)nh;
+    a.in_row = (long long)N1 * nh; a.out_row = (long long)N1 * nh;
+    a.in_field = p->nk; a.out_field = A_f;
+    a.in_member = 3 * p->nk; a.out_member = 6 * A_f;
+    a.n_fields = 6; a.n_members = B; a.inverse = 1;
+    a.mode = COL_CURL3;
+    a.gN0 = N0; a.gN1 = N1; a.gN2 = N2; a.gnh = (int)nh;
+    a.ks0 = p->ks[0]; a.ks1 = p->ks[1]; a.ks2 = p->ks[2];
+    a.mask23 = p->direct23;
+    PF_TRY(col<T>(p, M0, a, s));
+    // P2: axis-1 inverse, A -> Bbuf (w2)
+    a = col_args();
+    a.in = p->w1; a.out = p->w2;
+    a.rows_in = N1; a.rows_out = M1; a.cols = (int)nh;
+    a.n_outer = M0; a.in_outer = (long long)N1 * nh; a.out_outer = (long long)M1 * nh;
+    a.in_row = nh; a.out_row = nh;
+    a.in_field = A_f; a.out_field = B_f;
+    a.in_member = 6 * A_f; a.out_member = 6 * B_f;
+    a.n_fields = 6; a.n_members = B; a.inverse = 1;
+    PF_TRY(col<T>(p, M1, a, s));
+    // P3: last axis c2r, u x omega, r2c, truncate: Bbuf -> C (w1)
+    RowArgs r{};
+    r.in = p->w2; r.out = p->w1;
+    r.n_rows = M0 * M1; r.N = N2; r.nh = (int)nh;
+    r.in_row = nh; r.out_row = nh;
+    r.in_field = B_f; r.out_field = B_f;
+    r.in_member = 6 * B_f; r.out_member = 3 * B_f;
+    r.n_members = B; r.n_fields = 1;
+    r.mask23 = p->direct23; r.scale = 1.0 / M2;
+    PF_TRY(row<T>(p, M2, ROW_FUSED3, r, s));
+    // P4: axis-1 forward + truncation: C -> D (w2)
+    a = col_args();
+    a.in = p->w1; a.out = p->w2;
+    a.rows_in = M1; a.rows_out = N1; a.cols = (int)nh;
+    a.n_outer = M0; a.in_outer = (long long)M1 * nh; a.out_outer = (long long)N1 * nh;
+    a.in_row = nh; a.out_row = nh;
+    a.in_field = B_f; a.out_field = D_f;
+    a.in_member = 3 * B_f; a.out_member = 3 * D_f;
+    a.n_fields = 3; a.n_members = B; a.inverse = 0; a.scale = 1.0 / M1;
+    PF_TRY(col<T>(p, M1, a, s));
+    // P5: axis-0 forward + truncation: D -> R (w1)
+    a = col_args();
+    a.in = p->w2; a.out = p->w1;
+    a.rows_in = M0; a.rows_out = N0; a.cols = N1 * (int)nh;
+    a.in_row = (long long)N1 * nh; a.out_row = (long long)N1 * nh;
+    a.in_field = D_f; a.out_field = p->nk;
+    a.in_member = 3 * D_f; a.out_member = 3 * p->nk;
+    a.n_fields = 3; a.n_members = B; a.inverse = 0; a.scale = 1.0 / M0;
+    PF_TRY(col<T>(p, M0, a, s));
+  } else {
+    const int N0 = p->N[0], N1 = p->N[1];
+    const int M0 = p->M[0], M1 = p->M[1];
+    const long long A_f = (long long)M0 * nh;
+    // P1: axis-0 inverse with the scalar vorticity, u_hat -> A (3 fields, w1)
+    ColArgs a = col_args();
+    a.in = uhat; a.out = p->w1;
+    a.rows_in = N0; a.rows_out = M0; a.cols = (int)nh;
+    a.in_row = nh; a.out_row = nh;
+    a.in_field = p->nk; a.out_field = A_f;
+    a.in_member = 2 * p->nk; a.out_member = 3 * A_f;
+    a.n_fields = 3; a.n_members = B; a.inverse = 1;
+    a.mode = COL_CURL2;
+    a.gN0 = N0; a.gN1 = N1; a.gN2 = N1; a.gnh = (int)nh;
+    a.ks0 = p->ks[0]; a.ks1 = p->ks[1];
+    a.mask23 = p->direct23;
+    PF_TRY(col<T>(p, M0, a, s));
+    // P2: fused last axis: A -> C (2 fields, w2)
+    RowArgs r{};
+    r.in = p->w1; r.out = p->w2;
+    r.n_rows = M0; r.N = N1; r.nh = (int)nh;
+    r.in_row = nh; r.out_row = nh;
+    r.in_field = A_f; r.out_field = A_f;
+    r.in_member = 3 * A_f; r.out_member = 2 * A_f;
+    r.n_members = B; r.n_fields = 1;
+    r.mask23 = p->direct23; r.scale = 1.0 / M1;
+    PF_TRY(row<T>(p, M1, ROW_FUSED2, r, s));
+    // P3: axis-0 forward + truncation: C -> R (w1)
+    a = col_args();
+    a.in = p->w2; a.out = p->w1;
+    a.rows_in = M0; a.rows_out = N0; a.cols = (int)nh;
+    a.in_row = nh; a.out_row = nh;
+    a.in_field = A_f; a.out_field = p->nk;
+    a.in_member = 2 * A_f; a.out_member = 2 * p->nk;
+    a.n_fields = 2; a.n_members = B; a.inverse = 0; a.scale = 1.0 / M0;
+    PF_TRY(col<T>(p, M0, a, s));
+  }
+  return 0;
+}
+
+// Unpadded backward of nf fields (nf = members x components): in -> out.
+template <typename T> int backward_fields(pf_plan* p, const void* in, void* out, int nf, hipStream_t s) {
+  const long long nh = p->nh;
+  if ((size_t)nf * p->nk * p->csz > p->w1_bytes) return fail(PF_EINVAL, "backward: too many fields for workspace");
+  if (p->dim == 3) {
+    const int N0 = p->N[0], N1 = p->N[1], N2 = p->N[2];
+    ColArgs a = col_args();
+    a.in = in; a.out = p->w1;
+    a.rows_in = N0; a.rows_out = N0; a.cols = N1 * (int)nh;
+    a.in_row = (long long)N1 * nh; a.out_row = a.in_row;
+    a.in_field = p->nk; a.out_field = p->nk;
+    a.n_fields = nf; a.inverse = 1;
+    PF_TRY(col<T>(p, N0, a, s));
+    a = col_args();
+    a.in = p->w1; a.out = p->w1;
+    a.rows_in = N1; a.rows_out = N1; a.cols = (int)nh;
+    a.n_outer = N0; a.in_outer = (long long)N1 * nh; a.out_outer = a.in_outer;
+    a.in_row = nh; a.out_row = nh;
+    a.in_field = p->nk; a.out_field = p->nk;
+    a.n_fields = nf; a.inverse = 1;
+    PF_TRY(col<T>(p, N1, a, s));
+    RowArgs r{};
+    r.in = p->w1; r.out = out;
+    r.n_rows = N0 * N1; r.N = N2; r.nh = (int)nh;
+    r.in_row = nh; r.out_row = N2;
+    r.in_field = p->nk; r.out_field = p->nphys;
+    r.n_fields = nf; r.n_members = 1; r.out_scale = 1.0;
+    PF_TRY(row<T>(p, N2, ROW_C2R, r, s));
+  } else {
+    const int N0 = p->N[0], N1 = p->N[1];
+    ColArgs a = col_args();
+    a.in = in; a.out = p->w1;
+    a.rows_in = N0; a.rows_out = N0; a.cols = (int)nh;
+    a.in_row = nh; a.out_row = nh;
+    a.in_field = p->nk; a.out_field = p->nk;
+    a.n_fields = nf; a.inverse = 1;
+    PF_TRY(col<T>(p, N0, a, s));
+    RowArgs r{};
+    r.in = p->w1; r.out = out;
+    r.n_rows = N0; r.N = N1; r.nh = (int)nh;
+    r.in_row = nh; r.out_row = N1;
+    r.in_field = p->nk; r.out_field = p->nphys;
+    r.n_fields = nf; r.n_members = 1; r.out_scale = 1.0;
+    PF_TRY(row<T>(p, N1, ROW_C2R, r, s));
+  }
+  return 0;
+}
+
+template <typename T> int forward_fields(pf_plan* p, const void* in, void* out, int nf, hipStream_t s) {
+  const long long nh = p->nh;
+  if (p->dim == 3) {
+    const int N0 = p->N[0], N1 = p->N[1], N2 = p->N[2];
+    RowArgs r{};
+    r.in = in; r.out = out;
+    r.n_rows = N0 * N1; r.N = N2; r.nh = (int)nh;
+    r.in_row = N2; r.out_row = nh;
+    r.in_field = p->nphys; r.out_field = p->nk;
+    r.n_fields = nf; r.n_members = 1; r.scale = 1.0 / N2;
+    PF_TRY(row<T>(p, N2, ROW_R2C, r, s));
+    ColArgs a = col_args();
+    a.in = out; a.out = out;
+    a.rows_in = N1; a.rows_out = N1; a.cols = (int)nh;
+    a.n_outer = N0; a.in_outer = (long long)N1 * nh; a.out_outer = a.in_outer;
+    a.in_row = nh; a.out_row = nh;
+    a.in_field = p->nk; a.out_field = p->nk;
+    a.n_fields = nf; a.inverse = 0; a.scale = 1.0 / N1;
+    PF_TRY(col<T>(p, N1, a, s));
+    a = col_args();
+    a.in = out; a.out = out;
+    a.rows_in = N0; a.rows_out = N0; a.cols = N1 * (int)nh;
+    a.in_row = (long long)N1 * nh; a.out_row = a.in_row;
+    a.in_field = p->nk; a.out_field = p->nk;
+    a.n_fields = nf; a.inverse = 0; a.scale = 1.0 / N0;
+    PF_TRY(col<T>(p, N0, a, s));
+  } else {
+    const int N0 = p->N[0], N1 = p->N[1];
+    RowArgs r{};
+    r.in = in; r.out = out;
+    r.n_rows = N0; r.N = N1; r.nh = (int)nh;
+    r.in_row = N1; r.out_row = nh;
+    r.in_field = p->nphys; r.out_field = p->nk;
+    r.n_fields = nf; r.n_members = 1; r.scale = 1.0 / N1;
+    PF_TRY(row<T>(p, N1, ROW_R2C, r, s));
+    ColArgs a = col_args();
+    a.in = out; a.out = out;
+    a.rows_in = N0; a.rows_out = N0; a.cols = (int)nh;
+    a.in_row = nh; a.out_row = nh;
+    a.in_field = p->nk; a.out_field = p->nk;
+    a.n_fields = nf; a.inverse = 0; a.scale = 1.0 / N0;
+    PF_TRY(col<T>(p, N0, a, s));
+  }
+  return 0;
+}
+
+struct StageCoef {
+  double ca, cb, noise_a;
+  bool last;
+};
+
+int n_stages(int integ) { return integ == PF_RK3 ? 3 : (integ == PF_RK4 ? 4 : 2); }
+
+// integrator tables, time_integrator.py:45-47, :102-107, :170-174, :222-232, :287-288
+StageCoef stage_coef(int integ, int st) {
+  const double s2 = std::sqrt(2.0), s3 = std::sqrt(3.0);
+  switch (integ) {
+    case PF_EXPLICIT_PC: return {st == 0 ? 0.5 : 1.0, 0.0, st == 0 ? 0.0 : 1.0, st == 1};
+    case PF_IMPLICIT_PC: return {st == 0 ? 0.5 : 1.0, st == 0 ? 0.0 : 0.5, st == 0 ? 0.0 : 1.0, st == 1};
+    case PF_RK2: return {st == 0 ? 0.0 : 0.5, st == 0 ? 1.0 : 0.5, 0.0, st == 1};
+    case PF_RK3: {
+      const double a[3] = {0.0, 0.75, 1.0 / 3.0}, b[3] = {1.0, 0.25, 2.0 / 3.0};
+      const double nf[3] = {(2 * s2 + s3) / 5, (-4 * s2 + 3 * s3) / 5, (s2 - 2 * s3) / 10};
+      return {a[st], b[st], nf[st], st == 2};
+    }
+    default: {
+      const double a[4] = {0.25, 1.0 / 3.0, 0.5, 1.0};
+      return {a[st], 0.0, 0.0, st == 3};
+    }
+  }
+}
+
+bool uses_noise(int integ) { return integ == PF_EXPLICIT_PC || integ == PF_IMPLICIT_PC || integ == PF_RK3; }
+
+template <typename T> int band_energy(pf_plan* p, const void* uhat, hipStream_t s) {
+  BandArgs ba{};
+  ba.uhat = uhat; ba.coef = p->fcoef; ba.nk = p->nk;
+  ba.N0 = p->N[0]; ba.N1 = p->N[1]; ba.N2 = p->dim == 3 ? p->N[2] : p->N[1]; ba.nh = p->nh;
+  ba.ks0 = p->ks[0]; ba.ks1 = p->ks[1]; ba.ks2 = p->ks[2];
+  ba.kmin2 = p->d.forcing_kmin * p->d.forcing_kmin;
+  ba.kmax2 = p->d.forcing_kmax * p->d.forcing_kmax;
+  ba.kmax = p->d.forcing_kmax;
+  ba.power = p->d.forcing_power;
+  PF_TRY(launch_band<T>(p->dim, ba, p->B, s));
+  return 0;
+}
+
+// Arguments of one stage kernel launch (noise: raw draws of this step or null).
+StageArgs stage_args(const pf_plan* p, void* uhat, int stage, const void* noise, bool noise_on) {
+  const StageCoef c = stage_coef(p->d.integrator, stage);
+  StageArgs a = geometry(p);
+  a.uhat = uhat;
+  a.u0 = p->u0;
+  a.rhs_nl = p->d.nonlinear ? p->w1 : nullptr;
+  a.lin_op = p->lin_op;
+  a.corr = p->corr;
+  a.noise = noise_on ? noise : nullptr;
+  a.seeds = p->seeds;
+  a.fcoef = p->d.forcing == PF_FORCING_CONSTANT_POWER ? p->fcoef : nullptr;
+  a.integ = p->d.integrator;
+  a.stage = stage;
+  a.last_stage = c.last;
+  a.save_u0 = stage == 0;
+  a.ca = c.ca;
+  a.cb = c.cb;
+  a.dt = p->d.dt;
+  a.dt_sqrt = p->d.integrator == PF_RK3 ? std::sqrt(p->d.dt) : std::sqrt(0.5 * p->d.dt);
+  a.noise_on = noise_on;
+  a.noise_type = p->d.noise_type;
+  a.use_rng = noise == nullptr;
+  a.noise_a = c.noise_a;
+  a.noise_mag = p->d.noise_mag;
+  a.noise_norm = 1.0 / std::sqrt((double)p->nphys) / std::sqrt(2.0 - 2.0 / p->dim);
+  a.mask_nyq = p->d.mask_nyquist;
+  a.fk2min = p->d.forcing_kmin * p->d.forcing_kmin;
+  a.fk2max = p->d.forcing_kmax * p->d.forcing_kmax;
+  a.step = p->step;
+  a.mode = 0;
+  return a;
+}
+
+template <typename T> int step_impl(pf_plan* p, void* uhat, long long n_steps, const void* noise, hipStream_t s) {
+  const int ns = n_stages(p->d.integrator);
+  const bool noise_on = p->d.noise_mag != 0.0 && uses_noise(p->d.integrator);
+  const long long noise_block = (long long)p->B * 2 * p->nc * p->nk;  // complex per step
+  for (long long st = 0; st < n_steps; ++st) {
+    const void* nz = noise ? (const char*)noise + (size_t)(st * noise_block) * p->csz : nullptr;
+    for (int stage = 0; stage < ns; ++stage) {
+      if (p->d.nonlinear) {
+        int rc = nonlinear_unprojected<T>(p, uhat, s);
+        if (rc) return rc;
+      }
+      if (p->d.forcing == PF_FORCING_CONSTANT_POWER) {
+        int rc = band_energy<T>(p, uhat, s);
+        if (rc) return rc;
+      }
+      const StageArgs a = stage_args(p, uhat, stage, nz, noise_on);
+      PF_TRY(launch_stage<T>(p->dim, a, s));
+    }
+    p->step += 1;
+  }
+  return 0;
+}
+
+template <typename T>
+int compute_rhs_impl(pf_plan* p, void* uhat, double na, const void* noise, void* rnl, void* rlin, void* rf, void* rn,
+                     hipStream_t s) {
+  if (p->d.nonlinear) {
+    int rc = nonlinear_unprojected<T>(p, uhat, s);
+    if (rc) return rc;
+  }
+  if (p->d.forcing == PF_FORCING_CONSTANT_POWER) {
+    int rc = band_energy<T>(p, uhat, s);
+    if (rc) return rc;
+  }
+  StageArgs a = stage_args(p, uhat, 0, noise, p->d.noise_mag != 0.0);
+  a.noise_a = na;
+  a.mode = 1;
+  a.t_nl = rnl; a.t_lin = rlin; a.t_force = rf; a.t_noise = rn;
+  PF_TRY(launch_stage<T>(p->dim, a, s));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pf_last_error(void) { return g_err.c_str(); }
+const char* pf_version(void) { return "periodicflow-hip 0.1 gfx950"; }
+
+int pf_plan_create(const pf_desc* desc, pf_plan** out) {
+  if (!desc || !out) return fail(PF_EINVAL, "null argument");
+  *out = nullptr;
+  const pf_desc& d = *desc;
+  if (d.dim != 2 && d.dim != 3) return fail(PF_EINVAL, "NS solver only supports 2D and 3D problems.");
+  if (d.precision != PF_FP32 && d.precision != PF_FP64) return fail(PF_EINVAL, "precision must be PF_FP32 or PF_FP64");
+  if (d.dealias != PF_DEALIAS_PAD32 && d.dealias != PF_DEALIAS_TRUNC23) return fail(PF_EINVAL, "invalid dealiasing rule");
+  if (d.integrator < PF_EXPLICIT_PC || d.integrator > PF_RK4) return fail(PF_EINVAL, "invalid time integrator");
+  if (d.noise_type != PF_NOISE_THERMAL && d.noise_type != PF_NOISE_CORRELATED)
+    return fail(PF_EINVAL, "Invalid noise type. Options are 'thermal' or 'correlated'.");
+  if (d.forcing != PF_FORCING_NONE && d.forcing != PF_FORCING_CONSTANT_POWER) return fail(PF_EINVAL, "invalid forcing");
+  if (d.batch < 1) return fail(PF_EINVAL, "batch must be >= 1");
+  if (!(d.dt > 0)) return fail(PF_EINVAL, "dt must be > 0");
+  pf_plan* p = new pf_plan();
+  p->d = d;
+  p->dim = d.dim;
+  p->B = d.batch;
+  p->fp64 = d.precision == PF_FP64;
+  p->rsz = p->fp64 ? 8 : 4;
+  p->csz = 2 * p->rsz;
+  p->direct23 = d.dealias == PF_DEALIAS_TRUNC23;
+  for (int i = 0; i < d.dim; ++i) {
+    const int n = d.n[i];
+    if (n < 4 || n % 2) { delete p; return fail(PF_EINVAL, "grid sizes must be even and >= 4"); }
+    p->N[i] = n;
+    p->M[i] = p->direct23 ? n : (3 * n) / 2;
+    if (!supported_length(p->N[i]) || !supported_length(p->M[i])) {
+      delete p;
+      return fail(PF_EUNSUPPORTED, "grid size " + std::to_string(n) + " (padded " + std::to_string(p->M[i]) +
+                                       ") is not a compiled transform length");
+    }
+    if (!(d.length[i] > 0)) { delete p; return fail(PF_EINVAL, "box lengths must be > 0"); }
+    p->ks[i] = 6.283185307179586476925286766559 / d.length[i];
+  }
+  if (p->M[d.dim - 1] % 2) { delete p; return fail(PF_EINVAL, "padded last axis must be even"); }
+  const int last = d.dim - 1;
+  p->nh = p->N[last] / 2 + 1;
+  p->nk = (long long)p->nh;
+  p->nphys = 1;
+  for (int i = 0; i < d.dim; ++i) p->nphys *= p->N[i];
+  for (int i = 0; i < last; ++i) p->nk *= p->N[i];
+  p->nc = d.noise_type == PF_NOISE_THERMAL ? d.dim * d.dim : d.dim;
+
+  const long long B = p->B, nh = p->nh;
+  long long w1c = B * d.dim * p->nk, w2c = 0;  // complex elements
+  if (d.nonlinear) {
+    if (d.dim == 3) {
+      const long long M0 = p->M[0], M1 = p->M[1], N0 = p->N[0], N1 = p->N[1];
+      const long long A = B * 6 * M0 * N1 * nh, Bb = B * 6 * M0 * M1 * nh, C = B * 3 * M0 * M1 * nh;
+      const long long D = B * 3 * M0 * N1 * nh, R = B * 3 * N0 * N1 * nh;
+      w1c = std::max(std::max(w1c, A), std::max(C, R));
+      w2c = std::max(Bb, D);
+    } else {
+      const long long M0 = p->M[0], N0 = p->N[0];
+      const long long A = B * 3 * M0 * nh, C = B * 2 * M0 * nh, R = B * 2 * N0 * nh;
+      w1c = std::max(w1c, std::max(A, R));
+      w2c = C;
+    }
+  }
+  int rc = 0;
+  p->w1_bytes = (size_t)w1c * p->csz;
+  p->w2_bytes = (size_t)w2c * p->csz;
+  if ((rc = dev_alloc(p, &p->w1, p->w1_bytes)) ||
+      (w2c && (rc = dev_alloc(p, &p->w2, p->w2_bytes))) ||
+      (rc = dev_alloc(p, &p->u0, (size_t)B * d.dim * p->nk * p->csz)) ||
+      (rc = dev_alloc(p, (void**)&p->seeds, (size_t)B * sizeof(unsigned long long))) ||
+      (rc = dev_alloc(p, (void**)&p->fcoef, (size_t)B * sizeof(double)))) {
+    release(p);
+    delete p;
+    return rc;
+  }
+  for (int i = 0; i < d.dim; ++i) {
+    if ((rc = make_twiddles(p, p->N[i])) || (rc = make_twiddles(p, p->M[i]))) {
+      release(p);
+      delete p;
+      return rc;
+    }
+  }
+  std::vector<unsigned long long> seeds(B);
+  for (long long i = 0; i < B; ++i) seeds[i] = (unsigned long long)i;
+  hipError_t e = hipMemcpy(p->seeds, seeds.data(), B * sizeof(unsigned long long), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    release(p);
+    delete p;
+    return hip_fail(e, "hipMemcpy seeds");
+  }
+  *out = p;
+  return PF_OK;
+}
+
+int pf_plan_destroy(pf_plan* p) {
+  if (!p) return PF_OK;
+  (void)hipDeviceSynchronize();
+  release(p);
+  delete p;
+  return PF_OK;
+}
+
+int pf_plan_get_info(const pf_plan* p, pf_plan_info* info) {
+  if (!p || !info) return fail(PF_EINVAL, "null argument");
+  info->n_fourier = p->nk;
+  info->n_physical = p->nphys;
+  for (int i = 0; i < 3; ++i) info->padded[i] = i < p->dim ? p->M[i] : 1;
+  info->noise_components = p->nc;
+  info->workspace_bytes = (int64_t)p->bytes;
+  info->state_bytes = (int64_t)p->B * p->dim * p->nk * p->csz;
+  return PF_OK;
+}
+
+int pf_set_seeds(pf_plan* p, const uint64_t* seeds) {
+  if (!p || !seeds) return fail(PF_EINVAL, "null argument");
+  PF_HIP(hipMemcpy(p->seeds, seeds, (size_t)p->B * sizeof(uint64_t), hipMemcpyHostToDevice));
+  return PF_OK;
+}
+
+int pf_set_step_counter(pf_plan* p, int64_t step) {
+  if (!p) return fail(PF_EINVAL, "null plan");
+  p->step = step;
+  return PF_OK;
+}
+
+int64_t pf_get_step_counter(const pf_plan* p) { return p ? p->step : -1; }
+
+int pf_set_linear_operator(pf_plan* p, const void* L, void* stream) {
+  if (!p) return fail(PF_EINVAL, "null plan");
+  if (!L) { p->lin_op = nullptr; return PF_OK; }
+  void* dst = nullptr;
+  for (void* q : p->owned)
+    if (q == p->lin_op) dst = q;
+  if (!dst) {
+    int rc = dev_alloc(p, &dst, (size_t)p->nk * p->rsz);
+    if (rc) return rc;
+  }
+  PF_HIP(hipMemcpyAsync(dst, L, (size_t)p->nk * p->rsz, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  p->lin_op = dst;
+  return PF_OK;
+}
+
+int pf_set_correlation(pf_plan* p, const void* C, void* stream) {
+  if (!p || !C) return fail(PF_EINVAL, "null argument");
+  if (!p->corr) {
+    int rc = dev_alloc(p, &p->corr, (size_t)p->nk * p->csz);
+    if (rc) return rc;
+  }
+  PF_HIP(hipMemcpyAsync(p->corr, C, (size_t)p->nk * p->csz, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  return PF_OK;
+}
+
+int pf_forward(pf_plan* p, const void* u, void* uhat, int32_t ncomp, void* stream) {
+  if (!p || !u || !uhat || ncomp < 1) return fail(PF_EINVAL, "invalid argument");
+  const int nf = p->B * ncomp;
+  return p->fp64 ? forward_fields<double>(p, u, uhat, nf, (hipStream_t)stream)
+                 : forward_fields<float>(p, u, uhat, nf, (hipStream_t)stream);
+}
+
+int pf_backward(pf_plan* p, const void* uhat, void* u, int32_t ncomp, void* stream) {
+  if (!p || !u || !uhat || ncomp < 1) return fail(PF_EINVAL, "invalid argument");
+  // the workspace holds batch * dim fields: transform in chunks
+  const long long cap = (long long)(p->w1_bytes / ((size_t)p->nk * p->csz));
+  const long long total = (long long)p->B * ncomp;
+  for (long long f0 = 0; f0 < total; f0 += cap) {
+    const int nf = (int)std::min(cap, total - f0);
+    const char* src = (const char*)uhat + (size_t)f0 * p->nk * p->csz;
+    char* dst = (char*)u + (size_t)f0 * p->nphys * p->rsz;
+    int rc = p->fp64 ? backward_fields<double>(p, src, dst, nf, (hipStream_t)stream)
+                     : backward_fields<float>(p, src, dst, nf, (hipStream_t)stream);
+    if (rc) return rc;
+  }
+  return PF_OK;
+}
+
+int pf_project(pf_plan* p, void* uhat, void* stream) {
+  if (!p || !uhat) return fail(PF_EINVAL, "invalid argument");
+  const StageArgs g = geometry(p);
+  const int rc = p->fp64 ? launch_project<double>(p->dim, g, uhat, uhat, 0, 0, p->B, (hipStream_t)stream)
+                         : launch_project<float>(p->dim, g, uhat, uhat, 0, 0, p->B, (hipStream_t)stream);
+  return rc ? launch_fail(rc, "project") : PF_OK;
+}
+
+int pf_mask_zero_mode(pf_plan* p, void* uhat, void* stream) {
+  if (!p || !uhat) return fail(PF_EINVAL, "invalid argument");
+  const StageArgs g = geometry(p);
+  const int rc = p->fp64 ? launch_project<double>(p->dim, g, uhat, uhat, 0, 2, p->B, (hipStream_t)stream)
+                         : launch_project<float>(p->dim, g, uhat, uhat, 0, 2, p->B, (hipStream_t)stream);
+  return rc ? launch_fail(rc, "mask_zero_mode") : PF_OK;
+}
+
+int pf_rhs_nonlinear(pf_plan* p, const void* uhat, void* rhs, void* stream) {
+  if (!p || !uhat || !rhs) return fail(PF_EINVAL, "invalid argument");
+  if (!p->d.nonlinear) return fail(PF_EINVAL, "plan was created with nonlinear = 0");
+  hipStream_t s = (hipStream_t)stream;
+  int rc = p->fp64 ? nonlinear_unprojected<double>(p, uhat, s) : nonlinear_unprojected<float>(p, uhat, s);
+  if (rc) return rc;
+  const StageArgs g = geometry(p);
+  rc = p->fp64 ? launch_project<double>(p->dim, g, p->w1, rhs, p->d.mask_nyquist, 0, p->B, s)
+               : launch_project<float>(p->dim, g, p->w1, rhs, p->d.mask_nyquist, 0, p->B, s);
+  return rc ? launch_fail(rc, "project") : PF_OK;
+}
+
+int pf_step(pf_plan* p, void* uhat, int64_t n_steps, const void* noise, void* stream) {
+  if (!p || !uhat || n_steps < 0) return fail(PF_EINVAL, "invalid argument");
+  if (p->d.noise_type == PF_NOISE_CORRELATED && p->d.noise_mag != 0.0 && uses_noise(p->d.integrator) && !p->corr)
+    return fail(PF_EINVAL, "correlated noise needs pf_set_correlation first");
+  return p->fp64 ? step_impl<double>(p, uhat, n_steps, noise, (hipStream_t)stream)
+                 : step_impl<float>(p, uhat, n_steps, noise, (hipStream_t)stream);
+}
+
+int pf_compute_rhs(pf_plan* p, void* uhat, double a, const void* noise, void* rhs_nonlinear, void* rhs_linear,
+                   void* forcing, void* noise_out, void* stream) {
+  if (!p || !uhat || !rhs_nonlinear || !rhs_linear || !forcing || !noise_out) return fail(PF_EINVAL, "invalid argument");
+  if (p->d.noise_type == PF_NOISE_CORRELATED && p->d.noise_mag != 0.0 && !p->corr)
+    return fail(PF_EINVAL, "correlated noise needs pf_set_correlation first");
+  return p->fp64 ? compute_rhs_impl<double>(p, uhat, a, noise, rhs_nonlinear, rhs_linear, forcing, noise_out,
+                                            (hipStream_t)stream)
+                 : compute_rhs_impl<float>(p, uhat, a, noise, rhs_nonlinear, rhs_linear, forcing, noise_out,
+                                           (hipStream_t)stream);
+}
+
+int pf_stage_update(pf_plan* p, int32_t stage, void* unew, const void* u0, const void* rhs_nonlinear,
+                    void* rhs_linear, const void* forcing, const void* noise, void* stream) {
+  if (!p || !unew || !u0 || !rhs_nonlinear || !rhs_linear || !forcing || !noise) return fail(PF_EINVAL, "invalid argument");
+  if (stage < 0 || stage >= n_stages(p->d.integrator)) return fail(PF_EINVAL, "stage out of range");
+  StageArgs a = stage_args(p, unew, stage, nullptr, false);
+  a.u0 = const_cast<void*>(u0);
+  a.save_u0 = 0;
+  a.mode = 2;
+  a.rhs_nl = nullptr;
+  a.t_nl = const_cast<void*>(rhs_nonlinear);
+  a.t_lin = rhs_linear;
+  a.t_force = const_cast<void*>(forcing);
+  a.t_noise = const_cast<void*>(noise);
+  const int rc = p->fp64 ? launch_stage<double>(p->dim, a, (hipStream_t)stream)
+                         : launch_stage<float>(p->dim, a, (hipStream_t)stream);
+  return rc ? launch_fail(rc, "stage_update") : PF_OK;
+}
+
+int pf_draw_noise(pf_plan* p, void* out, void* stream) {
+  if (!p || !out) return fail(PF_EINVAL, "invalid argument");
+  const int rc = p->fp64 ? launch_draw<double>(out, p->seeds, p->step, p->nk, p->nc, p->B, (hipStream_t)stream)
+                         : launch_draw<float>(out, p->seeds, p->step, p->nk, p->nc, p->B, (hipStream_t)stream);
+  return rc ? launch_fail(rc, "draw_noise") : PF_OK;
+}
+
+int pf_diagnostics(pf_plan* p, const void* uhat, double* out, void* stream) {
+  if (!p || !uhat || !out) return fail(PF_EINVAL, "invalid argument");
+  DiagArgs a{};
+  a.uhat = uhat; a.out = out; a.nk = p->nk;
+  a.N0 = p->N[0]; a.N1 = p->N[1]; a.N2 = p->dim == 3 ? p->N[2] : p->N[1]; a.nh = p->nh;
+  a.ks0 = p->ks[0]; a.ks1 = p->ks[1]; a.ks2 = p->ks[2];
+  const int rc = p->fp64 ? launch_diag<double>(p->dim, a, p->B, (hipStream_t)stream)
+                         : launch_diag<float>(p->dim, a, p->B, (hipStream_t)stream);
+  return rc ? launch_fail(rc, "diagnostics") : PF_OK;
+}
+
+int pf_spectrum(pf_plan* p, const void* uhat, double* out, int32_t nshell, void* stream) {
+  if (!p || !uhat || !out || nshell < 1) return fail(PF_EINVAL, "invalid argument");
+  DiagArgs a{};
+  a.uhat = uhat; a.out = out; a.nk = p->nk;
+  a.N0 = p->N[0]; a.N1 = p->N[1]; a.N2 = p->dim == 3 ? p->N[2] : p->N[1]; a.nh = p->nh;
+  a.ks0 = p->ks[0]; a.ks1 = p->ks[1]; a.ks2 = p->ks[2];
+  const int rc = p->fp64 ? launch_spectrum<double>(p->dim, a, p->B, nshell, (hipStream_t)stream)
+                         : launch_spectrum<float>(p->dim, a, p->B, nshell, (hipStream_t)stream);
+  return rc ? launch_fail(rc, "spectrum") : PF_OK;
+}
+
+int pf_accumulate_sk(pf_plan* p, const void* uhat, double* acc, void* stream) {
+  if (!p || !uhat || !acc) return fail(PF_EINVAL, "invalid argument");
+  const int rc = p->fp64 ? launch_sk<double>(p->dim, uhat, acc, p->nk, p->B, (hipStream_t)stream)
+                         : launch_sk<float>(p->dim, uhat, acc, p->nk, p->B, (hipStream_t)stream);
+  return rc ? launch_fail(rc, "accumulate_sk") : PF_OK;
+}
+
+}  // extern "C"

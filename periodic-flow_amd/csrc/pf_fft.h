@@ -1,0 +1,210 @@
+// Register butterflies and the LDS Stockham engine used by every FFT pass.
+//
+// A length-L transform (L = 2^a 3^b) runs as a sequence of radix passes
+// (radix 3 first, then 16/8/4/2).  Pass p with radix R and span Ns (product
+// of the earlier radices) reads element j + r L/R, multiplies it by
+// W_{Ns R}^{r (j mod Ns)}, does a radix-R DFT in registers and writes it to
+// (j / Ns) Ns R + (j mod Ns) + r Ns: the Stockham auto-sort order, so the
+// result comes out in natural order with no bit reversal.  Every thread
+// reads all of its butterflies of a pass into registers before the block
+// barrier and writes them after it, so a pass runs in place in LDS.
+#pragma once
+#include "pf_common.h"
+
+namespace pf {
+
+struct RadixPlan {
+  int n;
+  int r[6];
+};
+
+constexpr RadixPlan make_plan(int L) {
+  RadixPlan p{0, {0, 0, 0, 0, 0, 0}};
+  int a = 0, b = 0, n = L;
+  while (n % 2 == 0) { n /= 2; ++a; }
+  while (n % 3 == 0) { n /= 3; ++b; }
+  for (int i = 0; i < b; ++i) p.r[p.n++] = 3;
+  switch (a) {
+    case 1: p.r[p.n++] = 2; break;
+    case 2: p.r[p.n++] = 4; break;
+    case 3: p.r[p.n++] = 8; break;
+    case 4: p.r[p.n++] = 16; break;
+    case 5: p.r[p.n++] = 8; p.r[p.n++] = 4; break;
+    case 6: p.r[p.n++] = 8; p.r[p.n++] = 8; break;
+    case 7: p.r[p.n++] = 16; p.r[p.n++] = 8; break;
+    case 8: p.r[p.n++] = 16; p.r[p.n++] = 16; break;
+    case 9: p.r[p.n++] = 8; p.r[p.n++] = 8; p.r[p.n++] = 8; break;
+    case 10: p.r[p.n++] = 16; p.r[p.n++] = 8; p.r[p.n++] = 8; break;
+    case 11: p.r[p.n++] = 16; p.r[p.n++] = 16; p.r[p.n++] = 8; break;
+    default: break;
+  }
+  return p;
+}
+
+constexpr bool plan_ok(int L) {
+  int n = L;
+  while (n % 2 == 0) n /= 2;
+  while (n % 3 == 0) n /= 3;
+  int a = 0;
+  for (int m = L; m % 2 == 0; m /= 2) ++a;
+  return n == 1 && a >= 1 && a <= 11;
+}
+
+// cos(2 pi m / 16) and sin(2 pi m / 16), m = 0..15
+constexpr double kCos16[16] = {1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
+                               0.0, -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613,
+                               -1.0, -0.92387953251128675613, -0.70710678118654752440, -0.38268343236508977173,
+                               0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613};
+constexpr double kSin16[16] = {0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613,
+                               1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
+                               0.0, -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613,
+                               -1.0, -0.92387953251128675613, -0.70710678118654752440, -0.38268343236508977173};
+constexpr double kSqrtHalf = 0.70710678118654752440;
+constexpr double kSqrt3Half = 0.86602540378443864676;
+
+// multiply by W16^m = exp(S 2 pi i m / 16) for a compile-time m
+template <int S, int M, typename T> __device__ __forceinline__ cplx<T> w16(cplx<T> a) {
+  constexpr int m = M & 15;
+  if constexpr (m == 0) return a;
+  else if constexpr (m == 4) return mul_si<S, T>(a);
+  else if constexpr (m == 8) return mk<T>(-a.x, -a.y);
+  else if constexpr (m == 12) return mul_si<-S, T>(a);
+  else {
+    const T c = (T)kCos16[m], s = (T)(S * kSin16[m]);
+    return mk<T>(a.x * c - a.y * s, a.x * s + a.y * c);
+  }
+}
+
+template <int S, typename T> __device__ __forceinline__ void dft2(cplx<T>& a0, cplx<T>& a1) {
+  cplx<T> t = a0;
+  a0 = add<T>(t, a1);
+  a1 = sub<T>(t, a1);
+}
+
+template <int S, typename T>
+__device__ __forceinline__ void dft4(cplx<T>& a0, cplx<T>& a1, cplx<T>& a2, cplx<T>& a3) {
+  cplx<T> t0 = add<T>(a0, a2), t1 = sub<T>(a0, a2);
+  cplx<T> t2 = add<T>(a1, a3), t3 = mul_si<S, T>(sub<T>(a1, a3));
+  a0 = add<T>(t0, t2);
+  a2 = sub<T>(t0, t2);
+  a1 = add<T>(t1, t3);
+  a3 = sub<T>(t1, t3);
+}
+
+template <int S, typename T> __device__ __forceinline__ void dft3(cplx<T>* v) {
+  cplx<T> t = add<T>(v[1], v[2]);
+  cplx<T> d = scl<T>(mul_si<S, T>(sub<T>(v[1], v[2])), (T)kSqrt3Half);
+  cplx<T> m = mk<T>(v[0].x - (T)0.5 * t.x, v[0].y - (T)0.5 * t.y);
+  v[0] = add<T>(v[0], t);
+  v[1] = add<T>(m, d);
+  v[2] = sub<T>(m, d);
+}
+
+template <int S, typename T> __device__ __forceinline__ void dft8(cplx<T>* v) {
+  // even / odd split, two radix-4 DFTs, then W8^k on the odd half
+  dft4<S, T>(v[0], v[2], v[4], v[6]);
+  dft4<S, T>(v[1], v[3], v[5], v[7]);
+  cplx<T> o1 = w16<S, 2, T>(v[3]), o2 = w16<S, 4, T>(v[5]), o3 = w16<S, 6, T>(v[7]);
+  cplx<T> e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6], o0 = v[1];
+  v[0] = add<T>(e0, o0); v[4] = sub<T>(e0, o0);
+  v[1] = add<T>(e1, o1); v[5] = sub<T>(e1, o1);
+  v[2] = add<T>(e2, o2); v[6] = sub<T>(e2, o2);
+  v[3] = add<T>(e3, o3); v[7] = sub<T>(e3, o3);
+}
+
+template <int S, typename T> __device__ __forceinline__ void dft16(cplx<T>* v) {
+  // 4 x 4: Z_r = DFT4(a_r, a_{r+4}, a_{r+8}, a_{r+12}); twiddle W16^{r k'};
+  // y_{k' + 4 q} = DFT4 over r.
+  dft4<S, T>(v[0], v[4], v[8], v[12]);
+  dft4<S, T>(v[1], v[5], v[9], v[13]);
+  dft4<S, T>(v[2], v[6], v[10], v[14]);
+  dft4<S, T>(v[3], v[7], v[11], v[15]);
+  // v[r + 4 k'] holds Z_r[k']
+  v[5] = w16<S, 1, T>(v[5]);   v[9] = w16<S, 2, T>(v[9]);   v[13] = w16<S, 3, T>(v[13]);
+  v[6] = w16<S, 2, T>(v[6]);   v[10] = w16<S, 4, T>(v[10]); v[14] = w16<S, 6, T>(v[14]);
+  v[7] = w16<S, 3, T>(v[7]);   v[11] = w16<S, 6, T>(v[11]); v[15] = w16<S, 9, T>(v[15]);
+  dft4<S, T>(v[0], v[1], v[2], v[3]);
+  dft4<S, T>(v[4], v[5], v[6], v[7]);
+  dft4<S, T>(v[8], v[9], v[10], v[11]);
+  dft4<S, T>(v[12], v[13], v[14], v[15]);
+  // now v[4 k' + q] holds y_{k' + 4 q}; transpose the 4 x 4 index
+  cplx<T> t;
+  t = v[1]; v[1] = v[4]; v[4] = t;
+  t = v[2]; v[2] = v[8]; v[8] = t;
+  t = v[3]; v[3] = v[12]; v[12] = t;
+  t = v[6]; v[6] = v[9]; v[9] = t;
+  t = v[7]; v[7] = v[13]; v[13] = t;
+  t = v[11]; v[11] = v[14]; v[14] = t;
+}
+
+template <int R, int S, typename T> __device__ __forceinline__ void dft(cplx<T>* v) {
+  if constexpr (R == 2) dft2<S, T>(v[0], v[1]);
+  else if constexpr (R == 3) dft3<S, T>(v);
+  else if constexpr (R == 4) dft4<S, T>(v[0], v[1], v[2], v[3]);
+  else if constexpr (R == 8) dft8<S, T>(v);
+  else if constexpr (R == 16) dft16<S, T>(v);
+}
+
+// twiddle table entry m of exp(-2 pi i m / L), conjugated for the inverse
+template <int S, typename T> __device__ __forceinline__ cplx<T> twiddle(const cplx<T>* tw, int m) {
+  cplx<T> w = tw[m];
+  if constexpr (S > 0) w.y = -w.y;
+  return w;
+}
+
+// One Stockham pass and the passes after it (compile-time recursion).
+// Addr(f, e) gives the LDS index of element e of transform f.  FFAST maps
+// consecutive threads to consecutive transforms (column tiles); otherwise
+// to consecutive butterflies of one transform (contiguous rows).
+template <typename T, int L, int S, int NF, int NT, int P, int NS, bool FFAST, class Addr>
+__device__ __forceinline__ void stockham(cplx<T>* buf, const cplx<T>* tw, const Addr& addr) {
+  constexpr RadixPlan pl = make_plan(L);
+  if constexpr (P < pl.n) {
+    constexpr int R = pl.r[P];
+    constexpr int NB = L / R;
+    constexpr int TOT = NF * NB;
+    constexpr int K = (TOT + NT - 1) / NT;
+    cplx<T> v[K][R];
+#pragma unroll
+    for (int kk = 0; kk < K; ++kk) {
+      const int q = threadIdx.x + kk * NT;
+      if (TOT % NT == 0 || q < TOT) {
+        const int f = FFAST ? q % NF : q / NB;
+        const int j = FFAST ? q / NF : q % NB;
+        const int k = j % NS;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[kk][r] = buf[addr(f, j + r * NB)];
+        if constexpr (NS > 1) {
+#pragma unroll
+          for (int r = 1; r < R; ++r) v[kk][r] = mul<T>(v[kk][r], twiddle<S, T>(tw, r * k * (L / (NS * R))));
+        }
+        dft<R, S, T>(v[kk]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < K; ++kk) {
+      const int q = threadIdx.x + kk * NT;
+      if (TOT % NT == 0 || q < TOT) {
+        const int f = FFAST ? q % NF : q / NB;
+        const int j = FFAST ? q / NF : q % NB;
+        const int k = j % NS;
+        const int base = (j / NS) * NS * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) buf[addr(f, base + r * NS)] = v[kk][r];
+      }
+    }
+    __syncthreads();
+    stockham<T, L, S, NF, NT, P + 1, NS * R, FFAST, Addr>(buf, tw, addr);
+  }
+}
+
+// NF independent length-L transforms held in LDS.  Callers synchronise the
+// block before the call; the call ends with a block barrier.
+template <typename T, int L, int S, int NF, int NT, bool FFAST, class Addr>
+__device__ __forceinline__ void lds_fft(cplx<T>* buf, const cplx<T>* tw, const Addr& addr) {
+  static_assert(plan_ok(L), "FFT length must be 2^a 3^b with 1 <= a <= 11");
+  stockham<T, L, S, NF, NT, 0, 1, FFAST, Addr>(buf, tw, addr);
+}
+
+}  // namespace pf

@@ -1,0 +1,33 @@
+// Host-side launchers (one translation unit per kernel family and precision).
+#pragma once
+#include "pf_kernels.h"
+
+// Transform lengths compiled into the library: 2^a 3^b covering N and 3N/2
+// for N in {8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 512, 1024}.
+#define PF_LENGTHS(X) \
+  X(8) X(12) X(16) X(18) X(24) X(32) X(36) X(48) X(64) X(72) X(96) X(128) X(144) X(192) X(256) X(288) \
+  X(384) X(512) X(768) X(1024) X(1536)
+
+namespace pf {
+
+bool length_supported(int L);
+
+// Launch one column pass (FFT along a strided axis).  Fills a.n_tiles.
+template <typename T> int launch_col(int L, ColArgs a, const void* tw, hipStream_t s);
+// Launch one row pass (last axis) of the given RowMode.
+template <typename T> int launch_row(int L, int mode, const RowArgs& a, const void* tw, hipStream_t s);
+
+template <typename T> int launch_stage(int dim, const StageArgs& a, hipStream_t s);
+template <typename T> int launch_diag(int dim, const DiagArgs& a, int n_members, hipStream_t s);
+template <typename T> int launch_spectrum(int dim, const DiagArgs& a, int n_members, int nshell, hipStream_t s);
+template <typename T> int launch_band(int dim, const BandArgs& a, int n_members, hipStream_t s);
+template <typename T> int launch_sk(int dim, const void* u, double* acc, long long nk, int nmem, hipStream_t s);
+template <typename T>
+int launch_project(int dim, const StageArgs& geo, const void* in, void* out, int mask_nyq, int zero_k0,
+                   int n_members, hipStream_t s);
+int launch_fill(double* p, long long n, double v, hipStream_t s);
+template <typename T>
+int launch_draw(void* out, const unsigned long long* seeds, long long step, long long nk, int nc, int nmem,
+                hipStream_t s);
+
+}  // namespace pf

@@ -1,0 +1,117 @@
+// Point-wise k-space kernels: stage update, projection, FP64 reductions.
+#include "pf_launch.h"
+
+namespace pf {
+
+__global__ void fill_kernel(double* p, long long n, double v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Leray projection (math.py:72-100) with optional Nyquist mask and k = 0 mask;
+// in may equal out.
+template <typename T, int D>
+__global__ void __launch_bounds__(kThreads)
+project_kernel(StageArgs g, const void* in, void* out, int mask_nyq, int zero_k0) {
+  const long long idx = (long long)blockIdx.x * kThreads + threadIdx.x;
+  const int b = blockIdx.y;
+  if (idx >= g.nk) return;
+  const Mode<T, D> m = decode<T, D>(g, idx);
+  const long long mb = (long long)b * D * g.nk + idx;
+  const cplx<T>* src = (const cplx<T>*)in + mb;
+  cplx<T>* dst = (cplx<T>*)out + mb;
+  cplx<T> v[D];
+#pragma unroll
+  for (int i = 0; i < D; ++i) v[i] = src[i * g.nk];
+  if (zero_k0 < 2) project<T, D>(v, m);
+  const bool kill = (mask_nyq && m.nyq) || (zero_k0 && m.k2 == (T)0);
+#pragma unroll
+  for (int i = 0; i < D; ++i) dst[i * g.nk] = kill ? mk<T>(0, 0) : v[i];
+}
+
+static dim3 mode_grid(long long nk, int nmem) {
+  return dim3((unsigned)((nk + kThreads - 1) / kThreads), (unsigned)nmem);
+}
+
+template <typename T> int launch_stage(int dim, const StageArgs& a, hipStream_t s) {
+  if (dim == 3) hipLaunchKernelGGL((stage_kernel<T, 3>), mode_grid(a.nk, a.n_members), dim3(kThreads), 0, s, a);
+  else hipLaunchKernelGGL((stage_kernel<T, 2>), mode_grid(a.nk, a.n_members), dim3(kThreads), 0, s, a);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+template <typename T>
+int launch_project(int dim, const StageArgs& g, const void* in, void* out, int mask_nyq, int zero_k0, int nmem,
+                   hipStream_t s) {
+  if (dim == 3)
+    hipLaunchKernelGGL((project_kernel<T, 3>), mode_grid(g.nk, nmem), dim3(kThreads), 0, s, g, in, out, mask_nyq,
+                       zero_k0);
+  else
+    hipLaunchKernelGGL((project_kernel<T, 2>), mode_grid(g.nk, nmem), dim3(kThreads), 0, s, g, in, out, mask_nyq,
+                       zero_k0);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+static unsigned red_blocks(long long nk) {
+  long long b = (nk + kThreads - 1) / kThreads;
+  if (b > 1024) b = 1024;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+
+int launch_fill(double* p, long long n, double v, hipStream_t s) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+template <typename T> int launch_diag(int dim, const DiagArgs& a, int nmem, hipStream_t s) {
+  int rc = launch_fill(a.out, 2LL * nmem, 0.0, s);
+  if (rc) return rc;
+  dim3 grid(red_blocks(a.nk), (unsigned)nmem);
+  if (dim == 3) hipLaunchKernelGGL((diag_kernel<T, 3>), grid, dim3(kThreads), 0, s, a);
+  else hipLaunchKernelGGL((diag_kernel<T, 2>), grid, dim3(kThreads), 0, s, a);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+template <typename T> int launch_spectrum(int dim, const DiagArgs& a, int nmem, int nshell, hipStream_t s) {
+  int rc = launch_fill(a.out, (long long)nshell * nmem, 0.0, s);
+  if (rc) return rc;
+  dim3 grid(red_blocks(a.nk), (unsigned)nmem);
+  if (dim == 3) hipLaunchKernelGGL((spectrum_kernel<T, 3>), grid, dim3(kThreads), 0, s, a, nshell);
+  else hipLaunchKernelGGL((spectrum_kernel<T, 2>), grid, dim3(kThreads), 0, s, a, nshell);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+template <typename T> int launch_band(int dim, const BandArgs& a, int nmem, hipStream_t s) {
+  if (dim == 3) hipLaunchKernelGGL((band_kernel<T, 3>), dim3((unsigned)nmem), dim3(kThreads), 0, s, a);
+  else hipLaunchKernelGGL((band_kernel<T, 2>), dim3((unsigned)nmem), dim3(kThreads), 0, s, a);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+template <typename T> int launch_sk(int dim, const void* u, double* acc, long long nk, int nmem, hipStream_t s) {
+  dim3 grid((unsigned)((nk + kThreads - 1) / kThreads));
+  if (dim == 3) hipLaunchKernelGGL((sk_kernel<T, 3>), grid, dim3(kThreads), 0, s, u, acc, nk, nmem);
+  else hipLaunchKernelGGL((sk_kernel<T, 2>), grid, dim3(kThreads), 0, s, u, acc, nk, nmem);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+template <typename T>
+int launch_draw(void* out, const unsigned long long* seeds, long long step, long long nk, int nc, int nmem,
+                hipStream_t s) {
+  dim3 grid((unsigned)((nk + 255) / 256), (unsigned)nmem);
+  hipLaunchKernelGGL((draw_kernel<T>), grid, dim3(256), 0, s, out, seeds, step, nk, nc);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+#define PF_INST(T)                                                                                    \
+  template int launch_stage<T>(int, const StageArgs&, hipStream_t);                                   \
+  template int launch_project<T>(int, const StageArgs&, const void*, void*, int, int, int, hipStream_t); \
+  template int launch_diag<T>(int, const DiagArgs&, int, hipStream_t);                                \
+  template int launch_spectrum<T>(int, const DiagArgs&, int, int, hipStream_t);                       \
+  template int launch_band<T>(int, const BandArgs&, int, hipStream_t);                                \
+  template int launch_sk<T>(int, const void*, double*, long long, int, hipStream_t);                  \
+  template int launch_draw<T>(void*, const unsigned long long*, long long, long long, int, int, hipStream_t);
+PF_INST(float)
+PF_INST(double)
+#undef PF_INST
+
+}  // namespace pf

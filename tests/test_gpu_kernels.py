@@ -1,0 +1,201 @@
+"""GPU parity of the HIP kernels against the FP64 oracle, through the C ABI.
+
+Bars: FP64 relative L2 <= 1e-12; FP32 relative L2 <= 1e-5 (stated per test).
+Inputs are seeded random spectra that fill every stored mode, including the
+Nyquist planes and the non-Hermitian kz = 0 plane that the reference's noise
+creates, so the padding / truncation / c2r conventions are all exercised.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import pf_oracle as po
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+FP32_TOL = 1e-5
+
+
+def _say(*a):
+    print("[gpu]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def pf():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from periodicflow.plan import Plan  # noqa: F401  (loads libpfhip.so; raises if missing)
+    import periodicflow.plan as m
+    return m
+
+
+def _rand_spec(rs, shape):
+    return rs.standard_normal(shape) + 1j * rs.standard_normal(shape)
+
+
+def _dev(x, cdt):
+    return torch.from_numpy(np.ascontiguousarray(x)).to(device="cuda", dtype=cdt)
+
+
+def _host(t):
+    return t.detach().cpu().numpy()
+
+
+@pytest.mark.parametrize("N", [(16, 24), (8, 12, 16), (12, 12, 8)])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_forward_backward(pf, N, prec):
+    _say("forward/backward", N, prec)
+    g = po.Grid(N)
+    plan = pf.Plan(N, 0.01, precision=prec, nonlinear=False, batch=2)
+    rs = np.random.RandomState(0)
+    u = rs.standard_normal((2, len(N)) + g.shape_physical)
+    ud = torch.from_numpy(u).to("cuda", plan.real_dtype)
+    uh = plan.new_state()
+    plan.forward(ud, uh)
+    ref = g.forward(u)
+    tol = FP64_TOL if prec == "fp64" else FP32_TOL
+    assert po.rel_l2(_host(uh), ref) < tol
+    # backward of a full random spectrum (non-Hermitian planes included)
+    sh = _rand_spec(rs, (2, len(N)) + g.shape_fourier)
+    back = torch.empty_like(ud)
+    plan.backward(_dev(sh, plan.complex_dtype), back)
+    refb = np.stack([g.backward(sh[b]) for b in range(2)])
+    assert po.rel_l2(_host(back), refb) < tol
+
+
+@pytest.mark.parametrize("N,dealias", [((16, 16), "3/2"), ((16, 24), "2/3"), ((24, 16), "3/2"),
+                                       ((8, 12, 16), "3/2"), ((12, 8, 8), "2/3"), ((16, 16, 16), "3/2")])
+@pytest.mark.parametrize("mask_nyquist", [False, True])
+def test_rhs_nonlinear_fp64(pf, N, dealias, mask_nyquist):
+    _say("rhs_nonlinear", N, dealias, mask_nyquist)
+    o = po.OracleSolver(N, 0.01, dealias=dealias, mask_nyquist=mask_nyquist)
+    plan = pf.Plan(N, 0.01, precision="fp64", dealias=dealias, mask_nyquist=mask_nyquist, batch=2)
+    rs = np.random.RandomState(1)
+    uh = _rand_spec(rs, (2, len(N)) + o.g.shape_fourier)
+    out = plan.new_state()
+    plan.rhs_nonlinear(_dev(uh, plan.complex_dtype), out)
+    ref = np.stack([o.rhs_nonlinear(uh[b]) for b in range(2)])
+    err = po.rel_l2(_host(out), ref)
+    _say("  rel err", err)
+    assert err < FP64_TOL
+
+
+CASES = [
+    # (N, integrator, nonlinear, noise_type, noise_mag, dealias)
+    ((16, 16), "explicit_pc", True, "thermal", 0.3, "3/2"),
+    ((16, 16), "implicit_pc", False, "thermal", 0.5, "3/2"),
+    ((16, 24), "rk2", True, "thermal", 0.0, "2/3"),
+    ((16, 16), "rk3", True, "correlated", 0.2, "3/2"),
+    ((16, 16), "rk4", True, "thermal", 0.0, "3/2"),
+    ((8, 8, 12), "rk3", True, "thermal", 0.05, "3/2"),
+    ((8, 12, 8), "implicit_pc", True, "correlated", 0.1, "2/3"),
+    ((12, 8, 8), "explicit_pc", False, "thermal", 0.2, "3/2"),
+    ((8, 8, 8), "rk4", True, "thermal", 0.0, "2/3"),
+    ((8, 8, 8), "rk2", True, "thermal", 0.0, "3/2"),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[1]}-{len(c[0])}d-{c[3]}-{c[5]}" for c in CASES])
+def test_step_parity_fp64(pf, case):
+    N, integ, nonlin, ntype, nmag, dealias = case
+    _say("step", case)
+    nsteps, B, nu, dt = 3, 2, 0.05, 0.02
+    plan = pf.Plan(N, dt, precision="fp64", integrator=integ, nonlinear=nonlin, viscosity=nu,
+                   noise_type=ntype, noise_mag=nmag, dealias=dealias, batch=B)
+    rs = np.random.RandomState(2)
+    refs, noises = [], []
+    u0 = None
+    states = []
+    for b in range(B):
+        o = po.OracleSolver(N, dt, is_nonlinear=nonlin, time_integrator=integ, viscosity=nu,
+                            noise_type=ntype, noise_mag=nmag, dealias=dealias, seed=100 + b)
+        if ntype == "correlated":
+            C = np.exp(-0.1 * o.g.k2) * (o.g.k2 > 0) + 0j
+            o.set_correlation_function(C)
+        uh = _rand_spec(rs, (len(N),) + o.g.shape_fourier) * 0.3
+        o.initialize(uh, space="fourier", project=True)
+        states.append(o.u_hat.copy())
+        steps = [o.noise_stream.next() for _ in range(nsteps)]
+        noises.append(steps)
+        o.solve(nsteps, noise=steps)
+        refs.append(o.u_hat)
+    if ntype == "correlated":
+        plan.set_correlation(_dev(C, plan.complex_dtype))
+    raw = np.zeros((nsteps, B, 2, plan.noise_components) + plan.shape_fourier, dtype=complex)
+    for b in range(B):
+        for s in range(nsteps):
+            raw[s, b, 0], raw[s, b, 1] = noises[b][s]
+    ud = _dev(np.stack(states), plan.complex_dtype)
+    plan.step(ud, nsteps, noise=_dev(raw, plan.complex_dtype))
+    err = po.rel_l2(_host(ud), np.stack(refs))
+    _say("  rel err", err)
+    assert err < FP64_TOL
+    assert plan.step_counter == nsteps
+
+
+def test_tgv3d_fp32_vs_fp64_oracle(pf):
+    N = (16, 16, 16)
+    _say("tgv fp32", N)
+    o = po.OracleSolver(N, 0.01, time_integrator="rk4", viscosity=1 / 400, dealias="2/3")
+    o.initialize(po.taylor_green_3d(o.g))
+    u_init = o.u_hat.copy()
+    o.solve(20)
+    plan = pf.Plan(N, 0.01, precision="fp32", integrator="rk4", viscosity=1 / 400, dealias="2/3")
+    ud = _dev(u_init[None], plan.complex_dtype)
+    plan.step(ud, 20)
+    err = po.rel_l2(_host(ud)[0], o.u_hat)
+    _say("  fp32 rel err", err)
+    assert err < FP32_TOL
+
+
+def test_diagnostics_and_spectrum(pf):
+    N = (12, 8, 16)
+    g = po.Grid(N)
+    plan = pf.Plan(N, 0.01, precision="fp64", nonlinear=False, batch=2)
+    rs = np.random.RandomState(3)
+    uh = _rand_spec(rs, (2, 3) + g.shape_fourier)
+    d = _host(plan.diagnostics(_dev(uh, plan.complex_dtype)))
+    for b in range(2):
+        assert np.isclose(d[b, 0], po.energy(g, uh[b]), rtol=1e-13)
+        assert np.isclose(d[b, 1], po.enstrophy(g, uh[b]), rtol=1e-13)
+    sp = _host(plan.spectrum(_dev(uh, plan.complex_dtype), 12))
+    for b in range(2):
+        assert np.allclose(sp[b], po.shell_spectrum(g, uh[b], 12), rtol=1e-12, atol=1e-14)
+
+
+def test_rng_noise_statistics(pf):
+    N = (64, 64)
+    plan = pf.Plan(N, 0.01, precision="fp64", nonlinear=False, batch=2, noise_mag=1.0, seeds=[7, 8])
+    out = torch.empty(plan.noise_shape(), dtype=plan.complex_dtype, device="cuda")
+    plan.draw_noise(out)
+    x = _host(out)
+    re = x.real.ravel()
+    im = x.imag.ravel()
+    assert abs(re.mean()) < 0.02 and abs(im.mean()) < 0.02
+    assert abs(re.var() - 1) < 0.03 and abs(im.var() - 1) < 0.03
+    # members differ, and the same (seed, step) redraws the same numbers
+    assert not np.allclose(x[0], x[1])
+    out2 = torch.empty_like(out)
+    plan.draw_noise(out2)
+    assert torch.equal(out, out2)
+
+
+def test_forcing_parity_fp64(pf):
+    N = (8, 8, 8)
+    _say("forcing")
+    dt = 0.01
+    o = po.OracleSolver(N, dt, time_integrator="rk3", viscosity=0.01, forcing="constant_power",
+                        forcing_power=0.1)
+    rs = np.random.RandomState(4)
+    o.initialize(_rand_spec(rs, (3,) + o.g.shape_fourier) * 0.2, space="fourier")
+    u_init = o.u_hat.copy()
+    o.solve(3)
+    plan = pf.Plan(N, dt, precision="fp64", integrator="rk3", viscosity=0.01, forcing="constant_power",
+                   forcing_power=0.1)
+    ud = _dev(u_init[None], plan.complex_dtype)
+    plan.step(ud, 3)
+    assert po.rel_l2(_host(ud)[0], o.u_hat) < FP64_TOL
