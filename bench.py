@@ -1,0 +1,187 @@
+#!/usr/bin/env python
+"""Mpoint-steps/s of the Fourier-Galerkin time step on one MI355X.
+
+Workload (one GPU, the plain single-instance configuration): 3D Taylor-Green
+vortex on a 256^3 periodic grid, nu = 1/1600, dt = 2.5e-3, low-storage RK4,
+2/3 truncation, FP32 arithmetic, synthetic initial field u = sin x cos y cos z,
+v = -cos x sin y cos z, w = 0.  A step is one full time step (four RK stages,
+each: nonlinear term through five FFT passes, then the fused stage update).
+
+Prints one JSON line.  ``value`` = 256^3 * steps / seconds / 1e6 with the
+state resident in HBM.  ``roofline`` prices the most expensive pass against
+the 8 TB/s HBM peak using the algorithmic bytes of periodicflow.perfmodel
+and the pass's mean duration from HIP events recorded around each of its
+launches inside the timed region.  ``cpu_baseline`` times the FP64 NumPy
+oracle (scipy.fft over all host cores) on a bounded sample of the same
+workload.  The FP32 result is compared with the FP64 GPU path after the same
+steps (``fp32_rel_l2_vs_fp64``).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "periodic-flow_amd"))
+
+N = 256
+NU = 1.0 / 1600.0
+DT = 2.5e-3
+
+
+def _tgv_state(plan, torch):
+    """TGV on the device: physical field -> forward transform -> projection."""
+    import math
+
+    n = plan.n[0]
+    x = torch.arange(n, dtype=plan.real_dtype, device="cuda") * (2 * math.pi / n)
+    X = x.view(n, 1, 1)
+    Y = x.view(1, n, 1)
+    Z = x.view(1, 1, n)
+    u = torch.zeros(plan.physical_shape(), dtype=plan.real_dtype, device="cuda")
+    u[0, 0] = torch.sin(X) * torch.cos(Y) * torch.cos(Z)
+    u[0, 1] = -torch.cos(X) * torch.sin(Y) * torch.cos(Z)
+    uh = plan.new_state()
+    plan.forward(u, uh)
+    plan.project(uh)
+    del u
+    return uh
+
+
+def cpu_baseline(steps):
+    from oracle import pf_oracle as po
+
+    cores = len(os.sched_getaffinity(0))
+    po.set_fft_workers(cores)
+    s = po.OracleSolver([N] * 3, DT, time_integrator="rk4", viscosity=NU, dealias="2/3")
+    s.initialize(po.taylor_green_3d(s.g))
+    t0 = time.perf_counter()
+    s.solve(steps)
+    dt = time.perf_counter() - t0
+    return {
+        "value": N**3 * steps / dt / 1e6,
+        "unit": "Mpoint-steps/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"FP64 NumPy/scipy.fft restatement of the reference path (CPU restatement standing in "
+                  f"for the reference mpirun path), TGV 256^3 RK4 2/3, {steps} step(s) incl. the "
+                  f"reference's per-step random_fields draws, {dt:.1f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64-check", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(members shard across GPUs with no exchange; see the design notes).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from periodicflow import perfmodel
+    from periodicflow.plan import Plan
+
+    torch.cuda.set_device(0)
+    plan = Plan([N] * 3, DT, precision="fp32", integrator="rk4", viscosity=NU, dealias="2/3")
+    u_init = _tgv_state(plan, torch)
+    uh = u_init.clone()
+    plan.step(uh, args.warmup)
+    uh.copy_(u_init)
+    torch.cuda.synchronize()
+
+    plan.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan.step(uh, args.steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = plan.profile_read()
+    plan.profile(False)
+    sec = t1 - t0
+
+    value = N**3 * args.steps / sec / 1e6
+    pb = perfmodel.pass_bytes([N] * 3, "2/3", "fp32")
+    per_pass = {}
+    for name, (ms, cnt) in prof.items():
+        if cnt:
+            avg = ms / cnt
+            per_pass[name] = {
+                "avg_us": round(avg * 1e3, 2),
+                "launches": cnt,
+                "share": round(ms / (sec * 1e3), 4),
+                "GBps": round(pb.get(name, 0) / (avg * 1e-3) / 1e9, 1) if name in pb else None,
+            }
+    dom = max(prof, key=lambda k: prof[k][0])
+    dom_avg_s = prof[dom][0] / prof[dom][1] * 1e-3
+    achieved = pb[dom] / dom_avg_s / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            t = json.load(f)
+        traffic = t.get(dom)
+    step_b = perfmodel.step_bytes([N] * 3, "2/3", "fp32", 4)
+
+    energy32 = float(plan.diagnostics(uh)[0, 0].item())
+    err = None
+    if not args.no_fp64_check:
+        p64 = Plan([N] * 3, DT, precision="fp64", integrator="rk4", viscosity=NU, dealias="2/3")
+        u64 = _tgv_state(p64, torch)
+        p64.step(u64, args.steps)
+        torch.cuda.synchronize()
+        diff = uh.to(torch.complex128) - u64
+        err = float(torch.linalg.vector_norm(diff) / torch.linalg.vector_norm(u64))
+        del u64, p64
+
+    out = {
+        "metric": "Mpoint-steps/s",
+        "value": round(value, 2),
+        "unit": "Mpoint-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sec / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (Taylor-Green initial field)",
+        "config": {
+            "workload": "tgv3d_256_rk4_trunc23_fp32",
+            "grid": [N, N, N],
+            "nu": NU,
+            "dt": DT,
+            "integrator": "rk4",
+            "dealias": "2/3",
+            "members": 1,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": round(achieved, 1),
+            "peak": perfmodel.HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / perfmodel.HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "algorithmic_bytes": pb[dom],
+        },
+        "step_roofline_frac": round(step_b / (sec / args.steps) / 1e9 / perfmodel.HBM_PEAK_GBS, 4),
+        "passes": per_pass,
+        "fp32_rel_l2_vs_fp64": err,
+        "energy_end": energy32,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -147,6 +147,26 @@ int pf_spectrum(pf_plan* plan, const void* u_hat, double* out, int32_t nshell, v
 /* acc (device doubles, nk) += sum over members and components |u_hat|^2. */
 int pf_accumulate_sk(pf_plan* plan, const void* u_hat, double* acc, void* stream);
 
+/* Per-pass timing with HIP events recorded on the launch stream around
+ * every launch of pf_step / pf_rhs_nonlinear while enabled.  Pass ids:
+ * PF_PASS_INV0 axis-0 inverse (+curl), PF_PASS_INV1 axis-1 inverse,
+ * PF_PASS_ROW last-axis c2r / cross product / r2c, PF_PASS_FWD1 axis-1
+ * forward, PF_PASS_FWD0 axis-0 forward, PF_PASS_STAGE stage update,
+ * PF_PASS_BAND forcing-band energy.  Enabling (or disabling) clears what
+ * was recorded; pf_profile_read synchronises and returns, per id, the
+ * summed milliseconds and the launch count.  Tooling; no counterpart in the
+ * reference. */
+#define PF_PASS_INV0 0
+#define PF_PASS_INV1 1
+#define PF_PASS_ROW 2
+#define PF_PASS_FWD1 3
+#define PF_PASS_FWD0 4
+#define PF_PASS_STAGE 5
+#define PF_PASS_BAND 6
+#define PF_NPASS 7
+int pf_profile_enable(pf_plan* plan, int32_t enable);
+int pf_profile_read(pf_plan* plan, double* total_ms, int64_t* count, int32_t n);
+
 const char* pf_last_error(void);
 const char* pf_version(void);
 

@@ -47,6 +47,29 @@ import math as _m
 
 import numpy as np
 
+class _FFT:
+    """FFT backend of the oracle: numpy.fft by default; scipy.fft with a
+    worker count for the timed CPU baseline (set_fft_workers)."""
+
+    workers = None
+
+    def __getattr__(self, name):
+        if self.workers is None:
+            return getattr(np.fft, name)
+        import functools
+
+        import scipy.fft as sf
+        return functools.partial(getattr(sf, name), workers=self.workers)
+
+
+_F = _FFT()
+
+
+def set_fft_workers(n):
+    """Use scipy.fft with ``n`` threads (None: numpy.fft, single threaded)."""
+    _F.workers = n
+
+
 SQRT2 = np.sqrt(2.0)
 SQRT3 = np.sqrt(3.0)
 
@@ -159,12 +182,12 @@ class Grid:
     def forward(self, u):
         """space_solver.py:215-219 -> shenfun V.forward: normalised rfftn."""
         axes = tuple(range(-self.dim, 0))
-        return np.fft.rfftn(u, axes=axes) / np.prod(self.N)
+        return _F.rfftn(u, axes=axes) / np.prod(self.N)
 
     def backward(self, uh):
         """space_solver.py:221-225 -> shenfun V.backward: unnormalised irfftn."""
         axes = tuple(range(-self.dim, 0))
-        return np.fft.irfftn(uh, s=self.N, axes=axes) * np.prod(self.N)
+        return _F.irfftn(uh, s=self.N, axes=axes) * np.prod(self.N)
 
     # ---- padded transforms (Vp.backward / Vp.forward) ----
     def _pad_c2c(self, a, ax, N, M):
@@ -227,7 +250,7 @@ class Grid:
             a = a * self.mask23
         for i in range(d - 1):
             a = self._pad_c2c(a, off + i, self.N[i], self.M[i])
-            a = np.fft.ifft(a, axis=off + i) * self.M[i]
+            a = _F.ifft(a, axis=off + i) * self.M[i]
         Nl, Ml = self.N[-1], self.M[-1]
         if Ml != Nl:
             shp = list(a.shape)
@@ -237,14 +260,14 @@ class Grid:
             b[..., :h] = a[..., :h]
             b[..., h] = 0.5 * a[..., h].real
             a = b
-        return np.fft.irfft(a, n=Ml, axis=-1) * Ml
+        return _F.irfft(a, n=Ml, axis=-1) * Ml
 
     def forward_padded(self, u):
         """Padded forward with truncation (Vp.forward, space_solver.py:296)."""
         d = self.dim
         off = u.ndim - d
         Nl, Ml = self.N[-1], self.M[-1]
-        a = np.fft.rfft(u, axis=-1) / Ml
+        a = _F.rfft(u, axis=-1) / Ml
         if Ml != Nl:
             h = Nl // 2
             b = np.zeros(a.shape[:-1] + (h + 1,), dtype=complex)
@@ -252,7 +275,7 @@ class Grid:
             b[..., h] = 2.0 * a[..., h].real
             a = b
         for i in reversed(range(d - 1)):
-            a = np.fft.fft(a, axis=off + i) / self.M[i]
+            a = _F.fft(a, axis=off + i) / self.M[i]
             a = self._trunc_c2c(a, off + i, self.N[i], self.M[i])
         return a
 

@@ -70,9 +70,33 @@ struct pf_plan {
   long long step = 0;
   size_t bytes = 0;
   std::vector<void*> owned;
+  // per-pass timing with HIP events (pf_profile_enable)
+  bool prof = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[PF_NPASS];
 };
 
 namespace {
+
+// Brackets one launch with a pair of events on its stream when profiling.
+struct PassTimer {
+  pf_plan* p;
+  int id;
+  hipStream_t s;
+  hipEvent_t a = nullptr, b = nullptr;
+  PassTimer(pf_plan* p_, int id_, hipStream_t s_) : p(p_), id(id_), s(s_) {
+    if (p && p->prof && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess) {
+      (void)hipEventRecord(a, s);
+    } else {
+      a = b = nullptr;
+    }
+  }
+  ~PassTimer() {
+    if (a && b) {
+      (void)hipEventRecord(b, s);
+      p->ev[id].emplace_back(a, b);
+    }
+  }
+};
 
 int dev_alloc(pf_plan* p, void** ptr, size_t bytes) {
   if (bytes == 0) bytes = 256;
@@ -181,7 +205,7 @@ template <typename T> int nonlinear_unprojected(pf_plan* p, const void* uhat, hi
     a.gN0 = N0; a.gN1 = N1; a.gN2 = N2; a.gnh = (int)nh;
     a.ks0 = p->ks[0]; a.ks1 = p->ks[1]; a.ks2 = p->ks[2];
     a.mask23 = p->direct23;
-    PF_TRY(col<T>(p, M0, a, s));
+    { PassTimer t(p, PF_PASS_INV0, s); PF_TRY(col<T>(p, M0, a, s)); }
     // P2: axis-1 inverse, A -> Bbuf (w2)
     a = col_args();
     a.in = p->w1; a.out = p->w2;
@@ -191,7 +215,7 @@ template <typename T> int nonlinear_unprojected(pf_plan* p, const void* uhat, hi
     a.in_field = A_f; a.out_field = B_f;
     a.in_member = 6 * A_f; a.out_member = 6 * B_f;
     a.n_fields = 6; a.n_members = B; a.inverse = 1;
-    PF_TRY(col<T>(p, M1, a, s));
+    { PassTimer t(p, PF_PASS_INV1, s); PF_TRY(col<T>(p, M1, a, s)); }
     // P3: last axis c2r, u x omega, r2c, truncate: Bbuf -> C (w1)
     RowArgs r{};
     r.in = p->w2; r.out = p->w1;
@@ -201,7 +225,7 @@ template <typename T> int nonlinear_unprojected(pf_plan* p, const void* uhat, hi
     r.in_member = 6 * B_f; r.out_member = 3 * B_f;
     r.n_members = B; r.n_fields = 1;
     r.mask23 = p->direct23; r.scale = 1.0 / M2;
-    PF_TRY(row<T>(p, M2, ROW_FUSED3, r, s));
+    { PassTimer t(p, PF_PASS_ROW, s); PF_TRY(row<T>(p, M2, ROW_FUSED3, r, s)); }
     // P4: axis-1 forward + truncation: C -> D (w2)
     a = col_args();
     a.in = p->w1; a.out = p->w2;
@@ -211,7 +235,7 @@ template <typename T> int nonlinear_unprojected(pf_plan* p, const void* uhat, hi
     a.in_field = B_f; a.out_field = D_f;
     a.in_member = 3 * B_f; a.out_member = 3 * D_f;
     a.n_fields = 3; a.n_members = B; a.inverse = 0; a.scale = 1.0 / M1;
-    PF_TRY(col<T>(p, M1, a, s));
+    { PassTimer t(p, PF_PASS_FWD1, s); PF_TRY(col<T>(p, M1, a, s)); }
     // P5: axis-0 forward + truncation: D -> R (w1)
     a = col_args();
     a.in = p->w2; a.out = p->w1;
@@ -220,7 +244,7 @@ template <typename T> int nonlinear_unprojected(pf_plan* p, const void* uhat, hi
     a.in_field = D_f; a.out_field = p->nk;
     a.in_member = 3 * D_f; a.out_member = 3 * p->nk;
     a.n_fields = 3; a.n_members = B; a.inverse = 0; a.scale = 1.0 / M0;
-    PF_TRY(col<T>(p, M0, a, s));
+    { PassTimer t(p, PF_PASS_FWD0, s); PF_TRY(col<T>(p, M0, a, s)); }
   } else {
     const int N0 = p->N[0], N1 = p->N[1];
     const int M0 = p->M[0], M1 = p->M[1];
@@ -237,7 +261,7 @@ template <typename T> int nonlinear_unprojected(pf_plan* p, const void* uhat, hi
     a.gN0 = N0; a.gN1 = N1; a.gN2 = N1; a.gnh = (int)nh;
     a.ks0 = p->ks[0]; a.ks1 = p->ks[1];
     a.mask23 = p->direct23;
-    PF_TRY(col<T>(p, M0, a, s));
+    { PassTimer t(p, PF_PASS_INV0, s); PF_TRY(col<T>(p, M0, a, s)); }
     // P2: fused last axis: A -> C (2 fields, w2)
     RowArgs r{};
     r.in = p->w1; r.out = p->w2;
@@ -247,7 +271,7 @@ template <typename T> int nonlinear_unprojected(pf_plan* p, const void* uhat, hi
     r.in_member = 3 * A_f; r.out_member = 2 * A_f;
     r.n_members = B; r.n_fields = 1;
     r.mask23 = p->direct23; r.scale = 1.0 / M1;
-    PF_TRY(row<T>(p, M1, ROW_FUSED2, r, s));
+    { PassTimer t(p, PF_PASS_ROW, s); PF_TRY(row<T>(p, M1, ROW_FUSED2, r, s)); }
     // P3: axis-0 forward + truncation: C -> R (w1)
     a = col_args();
     a.in = p->w2; a.out = p->w1;
@@ -256,7 +280,7 @@ template <typename T> int nonlinear_unprojected(pf_plan* p, const void* uhat, hi
     a.in_field = A_f; a.out_field = p->nk;
     a.in_member = 2 * A_f; a.out_member = 2 * p->nk;
     a.n_fields = 2; a.n_members = B; a.inverse = 0; a.scale = 1.0 / M0;
-    PF_TRY(col<T>(p, M0, a, s));
+    { PassTimer t(p, PF_PASS_FWD0, s); PF_TRY(col<T>(p, M0, a, s)); }
   }
   return 0;
 }
@@ -392,7 +416,7 @@ template <typename T> int band_energy(pf_plan* p, const void* uhat, hipStream_t 
   ba.kmax2 = p->d.forcing_kmax * p->d.forcing_kmax;
   ba.kmax = p->d.forcing_kmax;
   ba.power = p->d.forcing_power;
-  PF_TRY(launch_band<T>(p->dim, ba, p->B, s));
+  { PassTimer t(p, PF_PASS_BAND, s); PF_TRY(launch_band<T>(p->dim, ba, p->B, s)); }
   return 0;
 }
 
@@ -446,7 +470,7 @@ template <typename T> int step_impl(pf_plan* p, void* uhat, long long n_steps, c
         if (rc) return rc;
       }
       const StageArgs a = stage_args(p, uhat, stage, nz, noise_on);
-      PF_TRY(launch_stage<T>(p->dim, a, s));
+      { PassTimer t(p, PF_PASS_STAGE, s); PF_TRY(launch_stage<T>(p->dim, a, s)); }
     }
     p->step += 1;
   }
@@ -572,6 +596,7 @@ int pf_plan_create(const pf_desc* desc, pf_plan** out) {
 int pf_plan_destroy(pf_plan* p) {
   if (!p) return PF_OK;
   (void)hipDeviceSynchronize();
+  pf_profile_enable(p, 0);
   release(p);
   delete p;
   return PF_OK;
@@ -720,6 +745,33 @@ int pf_draw_noise(pf_plan* p, void* out, void* stream) {
   const int rc = p->fp64 ? launch_draw<double>(out, p->seeds, p->step, p->nk, p->nc, p->B, (hipStream_t)stream)
                          : launch_draw<float>(out, p->seeds, p->step, p->nk, p->nc, p->B, (hipStream_t)stream);
   return rc ? launch_fail(rc, "draw_noise") : PF_OK;
+}
+
+int pf_profile_enable(pf_plan* p, int32_t enable) {
+  if (!p) return fail(PF_EINVAL, "null plan");
+  for (auto& v : p->ev) {
+    for (auto& e : v) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    v.clear();
+  }
+  p->prof = enable != 0;
+  return PF_OK;
+}
+
+int pf_profile_read(pf_plan* p, double* total_ms, int64_t* count, int32_t n) {
+  if (!p || !total_ms || !count) return fail(PF_EINVAL, "null argument");
+  for (int i = 0; i < n; ++i) {
+    total_ms[i] = 0.0;
+    count[i] = 0;
+    if (i >= PF_NPASS) continue;
+    for (auto& e : p->ev[i]) {
+      PF_HIP(hipEventSynchronize(e.second));
+      float ms = 0.f;
+      PF_HIP(hipEventElapsedTime(&ms, e.first, e.second));
+      total_ms[i] += ms;
+      count[i] += 1;
+    }
+  }
+  return PF_OK;
 }
 
 int pf_diagnostics(pf_plan* p, const void* uhat, double* out, void* stream) {

@@ -3,6 +3,7 @@
 Same import name and Python API as the reference package
 (src/periodicflow/__init__.py, simulation.py, space_solver.py,
 time_integrator.py, basis.py, math.py); the array work runs in libpfhip.so.
+Submodules that need torch and the HIP library are imported on first use.
 """
 import logging
 

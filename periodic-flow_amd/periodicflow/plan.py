@@ -223,6 +223,18 @@ class Plan:
         call("pf_spectrum", self._h, _ptr(u_hat), _ptr(out), int(nshell), _stream())
         return out
 
+    def profile(self, enable=True):
+        """Start (or stop) per-pass HIP-event timing; clears earlier records."""
+        call("pf_profile_enable", self._h, int(bool(enable)))
+
+    def profile_read(self):
+        """{pass name: (total ms, launches)} of what was recorded since profile()."""
+        n = len(_lib.PASS_NAMES)
+        ms = (C.c_double * n)()
+        cnt = (C.c_int64 * n)()
+        call("pf_profile_read", self._h, ms, cnt, n)
+        return {nm: (float(ms[i]), int(cnt[i])) for i, nm in enumerate(_lib.PASS_NAMES)}
+
     def accumulate_sk(self, u_hat, acc):
         self._check(u_hat, self.state_shape(), self.complex_dtype, "u_hat")
         self._check(acc, self.shape_fourier, torch.float64, "acc")

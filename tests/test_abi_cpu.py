@@ -1,0 +1,106 @@
+"""C ABI checks that need no GPU: the library loads, exports every entry
+point include/periodicflow.h declares, and rejects invalid descriptors
+before touching the device."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "periodicflow.h")
+
+
+def _declared():
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(pf_\w+)\s*\(", txt, re.M)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from periodicflow import _lib
+
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libpfhip.so not built")
+    return _lib.load()
+
+
+def test_header_and_binding_agree(lib):
+    from periodicflow import _lib
+
+    declared = _declared()
+    assert len(declared) >= 20
+    assert sorted(_lib.EXPORTED) == declared
+    for name in declared:
+        assert hasattr(lib, name), name
+
+
+def test_version(lib):
+    assert b"gfx950" in lib.pf_version()
+
+
+def _desc(**kw):
+    from periodicflow import _lib
+
+    d = _lib.PfDesc()
+    d.dim = 3
+    for i in range(3):
+        d.n[i] = 16
+        d.length[i] = 6.283185307179586
+    d.precision = 0
+    d.dealias = 0
+    d.integrator = 3
+    d.nonlinear = 1
+    d.noise_type = 0
+    d.forcing = 0
+    d.batch = 1
+    d.viscosity = 0.01
+    d.dt = 0.01
+    for k, v in kw.items():
+        if k == "n":
+            for i, x in enumerate(v):
+                d.n[i] = x
+        else:
+            setattr(d, k, v)
+    return d
+
+
+@pytest.mark.parametrize("kw,code,msg", [
+    (dict(dim=4), -1, "2D and 3D"),
+    (dict(precision=7), -1, "precision"),
+    (dict(dealias=5), -1, "dealias"),
+    (dict(integrator=9), -1, "integrator"),
+    (dict(noise_type=3), -1, "noise type"),
+    (dict(batch=0), -1, "batch"),
+    (dict(dt=0.0), -1, "dt"),
+    (dict(n=(15, 16, 16)), -1, "even"),
+    (dict(n=(20, 16, 16)), -4, "transform length"),
+    (dict(n=(16, 16, 40)), -4, "transform length"),
+])
+def test_plan_create_rejects(lib, kw, code, msg):
+    h = C.c_void_p()
+    d = _desc(**kw)
+    rc = lib.pf_plan_create(C.byref(d), C.byref(h))
+    assert rc == code
+    assert msg in lib.pf_last_error().decode()
+    assert not h.value
+
+
+def test_null_arguments(lib):
+    assert lib.pf_plan_create(None, None) == -1
+    assert lib.pf_step(None, None, 1, None, None) == -1
+    assert lib.pf_get_step_counter(None) == -1
+    assert lib.pf_plan_destroy(None) == 0
+
+
+def test_perfmodel_matches_byte_table():
+    from periodicflow import perfmodel
+
+    # 256^3, 2/3 rule, FP32: nk = 256*256*129 modes; stage kernel moves 12 nk complex
+    pb = perfmodel.pass_bytes([256] * 3, "2/3", "fp32")
+    nk = 256 * 256 * 129
+    assert pb["stage"] == 12 * nk * 8
+    assert pb["inv_axis1"] == 12 * nk * 8
+    # 3/2 rule pads the inverse outputs by 3/2 per padded axis
+    pb32 = perfmodel.pass_bytes([64] * 3, "3/2", "fp64")
+    assert pb32["inv_axis0_curl"] == (3 * 64 * 64 * 33 + 6 * 96 * 64 * 33) * 16
