@@ -108,7 +108,8 @@ def main():
     sec = t1 - t0
 
     value = N**3 * args.steps / sec / 1e6
-    pb = perfmodel.pass_bytes([N] * 3, "2/3", "fp32")
+    fused = "fwd_axis0_stage" in {k for k, (ms, c) in prof.items() if c}
+    pb = perfmodel.pass_bytes([N] * 3, "2/3", "fp32", fused=fused)
     per_pass = {}
     for name, (ms, cnt) in prof.items():
         if cnt:
@@ -119,7 +120,7 @@ def main():
                 "share": round(ms / (sec * 1e3), 4),
                 "GBps": round(pb.get(name, 0) / (avg * 1e-3) / 1e9, 1) if name in pb else None,
             }
-    dom = max(prof, key=lambda k: prof[k][0])
+    dom = max((k for k in prof if k in pb), key=lambda k: prof[k][0])
     dom_avg_s = prof[dom][0] / prof[dom][1] * 1e-3
     achieved = pb[dom] / dom_avg_s / 1e9
     traffic = None
@@ -128,7 +129,7 @@ def main():
         with open(tf) as f:
             t = json.load(f)
         traffic = t.get(dom)
-    step_b = perfmodel.step_bytes([N] * 3, "2/3", "fp32", 4)
+    step_b = perfmodel.step_bytes([N] * 3, "2/3", "fp32", 4, fused=fused)
 
     energy32 = float(plan.diagnostics(uh)[0, 0].item())
     err = None

@@ -152,7 +152,8 @@ int pf_accumulate_sk(pf_plan* plan, const void* u_hat, double* acc, void* stream
  * PF_PASS_INV0 axis-0 inverse (+curl), PF_PASS_INV1 axis-1 inverse,
  * PF_PASS_ROW last-axis c2r / cross product / r2c, PF_PASS_FWD1 axis-1
  * forward, PF_PASS_FWD0 axis-0 forward, PF_PASS_STAGE stage update,
- * PF_PASS_BAND forcing-band energy.  Enabling (or disabling) clears what
+ * PF_PASS_BAND forcing-band energy, PF_PASS_FWD0_STAGE axis-0 forward
+ * fused with the stage update (pf_step's default).  Enabling (or disabling) clears what
  * was recorded; pf_profile_read synchronises and returns, per id, the
  * summed milliseconds and the launch count.  Tooling; no counterpart in the
  * reference. */
@@ -163,9 +164,14 @@ int pf_accumulate_sk(pf_plan* plan, const void* u_hat, double* acc, void* stream
 #define PF_PASS_FWD0 4
 #define PF_PASS_STAGE 5
 #define PF_PASS_BAND 6
-#define PF_NPASS 7
+#define PF_PASS_FWD0_STAGE 7
+#define PF_NPASS 8
 int pf_profile_enable(pf_plan* plan, int32_t enable);
 int pf_profile_read(pf_plan* plan, double* total_ms, int64_t* count, int32_t n);
+/* 1 (default): pf_step fuses the axis-0 forward pass with the stage update
+ * where the tile fits in LDS; 0: separate passes (the unfused sequence the
+ * per-stage API uses).  Results agree to rounding. */
+int pf_set_fusion(pf_plan* plan, int32_t fuse_stage);
 
 const char* pf_last_error(void);
 const char* pf_version(void);

@@ -72,6 +72,7 @@ struct pf_plan {
   std::vector<void*> owned;
   // per-pass timing with HIP events (pf_profile_enable)
   bool prof = false;
+  bool fuse_stage = true;   // axis-0 forward pass fused with the stage update
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[PF_NPASS];
 };
 
@@ -167,8 +168,8 @@ StageArgs geometry(const pf_plan* p) {
   return g;
 }
 
-template <typename T> int col(const pf_plan* p, int L, const ColArgs& a, hipStream_t s) {
-  return launch_col<T>(L, a, p->tw.at(L), s);
+template <typename T> int col(const pf_plan* p, int L, int kind, const ColArgs& a, hipStream_t s) {
+  return launch_col<T>(L, kind, a, p->tw.at(L), s);
 }
 template <typename T> int row(const pf_plan* p, int L, int mode, const RowArgs& a, hipStream_t s) {
   return launch_row<T>(L, mode, a, p->tw.at(L), s);
@@ -180,32 +181,39 @@ ColArgs col_args() {
   a.n_members = 1;
   a.n_fields = 1;
   a.scale = 1.0;
-  a.mode = COL_PLAIN;
   return a;
 }
 
-// Nonlinear term (unprojected) of u_hat into the plan's R buffer (w1).
-template <typename T> int nonlinear_unprojected(pf_plan* p, const void* uhat, hipStream_t s) {
+// Inverse passes, the fused last-axis pass and (3D) the axis-1 forward pass
+// of the nonlinear term: u_hat -> input of the axis-0 forward pass (w2),
+// laid out (B, d, M0, [N1,] nh).
+template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStream_t s) {
   const int B = p->B;
   const long long nh = p->nh;
+  const bool pr = p->direct23;
   if (p->dim == 3) {
     const int N0 = p->N[0], N1 = p->N[1], N2 = p->N[2];
     const int M0 = p->M[0], M1 = p->M[1], M2 = p->M[2];
     const long long A_f = (long long)M0 * N1 * nh, B_f = (long long)M0 * M1 * nh;
     const long long D_f = (long long)M0 * N1 * nh;
-    // P1: axis-0 inverse with curl, u_hat -> A (6 fields)
+    // P1: axis-0 inverse with curl, u_hat -> A (6 fields, w1)
     ColArgs a = col_args();
     a.in = uhat; a.out = p->w1;
     a.rows_in = N0; a.rows_out = M0; a.cols = N1 * (int)nh;
     a.in_row = (long long)N1 * nh; a.out_row = (long long)N1 * nh;
     a.in_field = p->nk; a.out_field = A_f;
     a.in_member = 3 * p->nk; a.out_member = 6 * A_f;
-    a.n_fields = 6; a.n_members = B; a.inverse = 1;
-    a.mode = COL_CURL3;
+    a.n_fields = 6; a.n_members = B;
     a.gN0 = N0; a.gN1 = N1; a.gN2 = N2; a.gnh = (int)nh;
     a.ks0 = p->ks[0]; a.ks1 = p->ks[1]; a.ks2 = p->ks[2];
-    a.mask23 = p->direct23;
-    { PassTimer t(p, PF_PASS_INV0, s); PF_TRY(col<T>(p, M0, a, s)); }
+    a.mask23 = pr;
+    if (pr) {
+      a.prune = 1;
+      a.K1 = N1 / 3;
+      a.n2k = N2 / 3 + 1;
+      a.cols = (2 * a.K1 + 1) * a.n2k;
+    }
+    { PassTimer t(p, PF_PASS_INV0, s); PF_TRY(col<T>(p, M0, K_INV_CURL3, a, s)); }
     // P2: axis-1 inverse, A -> Bbuf (w2)
     a = col_args();
     a.in = p->w1; a.out = p->w2;
@@ -214,8 +222,13 @@ template <typename T> int nonlinear_unprojected(pf_plan* p, const void* uhat, hi
     a.in_row = nh; a.out_row = nh;
     a.in_field = A_f; a.out_field = B_f;
     a.in_member = 6 * A_f; a.out_member = 6 * B_f;
-    a.n_fields = 6; a.n_members = B; a.inverse = 1;
-    { PassTimer t(p, PF_PASS_INV1, s); PF_TRY(col<T>(p, M1, a, s)); }
+    a.n_fields = 6; a.n_members = B;
+    if (pr) {
+      a.prune = 1;
+      a.Krow = N1 / 3;
+      a.cols = N2 / 3 + 1;
+    }
+    { PassTimer t(p, PF_PASS_INV1, s); PF_TRY(col<T>(p, M1, K_INV, a, s)); }
     // P3: last axis c2r, u x omega, r2c, truncate: Bbuf -> C (w1)
     RowArgs r{};
     r.in = p->w2; r.out = p->w1;
@@ -224,7 +237,7 @@ template <typename T> int nonlinear_unprojected(pf_plan* p, const void* uhat, hi
     r.in_field = B_f; r.out_field = B_f;
     r.in_member = 6 * B_f; r.out_member = 3 * B_f;
     r.n_members = B; r.n_fields = 1;
-    r.mask23 = p->direct23; r.scale = 1.0 / M2;
+    r.mask23 = pr; r.scale = 1.0 / M2;
     { PassTimer t(p, PF_PASS_ROW, s); PF_TRY(row<T>(p, M2, ROW_FUSED3, r, s)); }
     // P4: axis-1 forward + truncation: C -> D (w2)
     a = col_args();
@@ -234,17 +247,8 @@ template <typename T> int nonlinear_unprojected(pf_plan* p, const void* uhat, hi
     a.in_row = nh; a.out_row = nh;
     a.in_field = B_f; a.out_field = D_f;
     a.in_member = 3 * B_f; a.out_member = 3 * D_f;
-    a.n_fields = 3; a.n_members = B; a.inverse = 0; a.scale = 1.0 / M1;
-    { PassTimer t(p, PF_PASS_FWD1, s); PF_TRY(col<T>(p, M1, a, s)); }
-    // P5: axis-0 forward + truncation: D -> R (w1)
-    a = col_args();
-    a.in = p->w2; a.out = p->w1;
-    a.rows_in = M0; a.rows_out = N0; a.cols = N1 * (int)nh;
-    a.in_row = (long long)N1 * nh; a.out_row = (long long)N1 * nh;
-    a.in_field = D_f; a.out_field = p->nk;
-    a.in_member = 3 * D_f; a.out_member = 3 * p->nk;
-    a.n_fields = 3; a.n_members = B; a.inverse = 0; a.scale = 1.0 / M0;
-    { PassTimer t(p, PF_PASS_FWD0, s); PF_TRY(col<T>(p, M0, a, s)); }
+    a.n_fields = 3; a.n_members = B; a.scale = 1.0 / M1;
+    { PassTimer t(p, PF_PASS_FWD1, s); PF_TRY(col<T>(p, M1, K_FWD, a, s)); }
   } else {
     const int N0 = p->N[0], N1 = p->N[1];
     const int M0 = p->M[0], M1 = p->M[1];
@@ -256,12 +260,12 @@ template <typename T> int nonlinear_unprojected(pf_plan* p, const void* uhat, hi
     a.in_row = nh; a.out_row = nh;
     a.in_field = p->nk; a.out_field = A_f;
     a.in_member = 2 * p->nk; a.out_member = 3 * A_f;
-    a.n_fields = 3; a.n_members = B; a.inverse = 1;
-    a.mode = COL_CURL2;
+    a.n_fields = 3; a.n_members = B;
     a.gN0 = N0; a.gN1 = N1; a.gN2 = N1; a.gnh = (int)nh;
     a.ks0 = p->ks[0]; a.ks1 = p->ks[1];
-    a.mask23 = p->direct23;
-    { PassTimer t(p, PF_PASS_INV0, s); PF_TRY(col<T>(p, M0, a, s)); }
+    a.mask23 = pr;
+    if (pr) a.cols = N1 / 3 + 1;
+    { PassTimer t(p, PF_PASS_INV0, s); PF_TRY(col<T>(p, M0, K_INV_CURL2, a, s)); }
     // P2: fused last axis: A -> C (2 fields, w2)
     RowArgs r{};
     r.in = p->w1; r.out = p->w2;
@@ -270,18 +274,35 @@ template <typename T> int nonlinear_unprojected(pf_plan* p, const void* uhat, hi
     r.in_field = A_f; r.out_field = A_f;
     r.in_member = 3 * A_f; r.out_member = 2 * A_f;
     r.n_members = B; r.n_fields = 1;
-    r.mask23 = p->direct23; r.scale = 1.0 / M1;
+    r.mask23 = pr; r.scale = 1.0 / M1;
     { PassTimer t(p, PF_PASS_ROW, s); PF_TRY(row<T>(p, M1, ROW_FUSED2, r, s)); }
-    // P3: axis-0 forward + truncation: C -> R (w1)
-    a = col_args();
-    a.in = p->w2; a.out = p->w1;
-    a.rows_in = M0; a.rows_out = N0; a.cols = (int)nh;
-    a.in_row = nh; a.out_row = nh;
-    a.in_field = A_f; a.out_field = p->nk;
-    a.in_member = 2 * A_f; a.out_member = 2 * p->nk;
-    a.n_fields = 2; a.n_members = B; a.inverse = 0; a.scale = 1.0 / M0;
-    { PassTimer t(p, PF_PASS_FWD0, s); PF_TRY(col<T>(p, M0, a, s)); }
   }
+  return 0;
+}
+
+// Arguments of the axis-0 forward pass from the front's output (w2).
+ColArgs last_forward_args(const pf_plan* p) {
+  const long long nh = p->nh;
+  const int d = p->dim;
+  const int M0 = p->M[0], N0 = p->N[0];
+  const long long inner = d == 3 ? (long long)p->N[1] * nh : nh;
+  const long long D_f = (long long)M0 * inner;
+  ColArgs a = col_args();
+  a.in = p->w2; a.out = p->w1;
+  a.rows_in = M0; a.rows_out = N0; a.cols = (int)inner;
+  a.in_row = inner; a.out_row = inner;
+  a.in_field = D_f; a.out_field = p->nk;
+  a.in_member = d * D_f; a.out_member = d * p->nk;
+  a.n_fields = d; a.n_members = p->B; a.scale = 1.0 / M0;
+  return a;
+}
+
+// Nonlinear term (unprojected) of u_hat into the plan's R buffer (w1).
+template <typename T> int nonlinear_unprojected(pf_plan* p, const void* uhat, hipStream_t s) {
+  int rc = nonlinear_front<T>(p, uhat, s);
+  if (rc) return rc;
+  const ColArgs a = last_forward_args(p);
+  { PassTimer t(p, PF_PASS_FWD0, s); PF_TRY(col<T>(p, p->M[0], K_FWD, a, s)); }
   return 0;
 }
 
@@ -296,16 +317,16 @@ template <typename T> int backward_fields(pf_plan* p, const void* in, void* out,
     a.rows_in = N0; a.rows_out = N0; a.cols = N1 * (int)nh;
     a.in_row = (long long)N1 * nh; a.out_row = a.in_row;
     a.in_field = p->nk; a.out_field = p->nk;
-    a.n_fields = nf; a.inverse = 1;
-    PF_TRY(col<T>(p, N0, a, s));
+    a.n_fields = nf;
+    PF_TRY(col<T>(p, N0, K_INV, a, s));
     a = col_args();
     a.in = p->w1; a.out = p->w1;
     a.rows_in = N1; a.rows_out = N1; a.cols = (int)nh;
     a.n_outer = N0; a.in_outer = (long long)N1 * nh; a.out_outer = a.in_outer;
     a.in_row = nh; a.out_row = nh;
     a.in_field = p->nk; a.out_field = p->nk;
-    a.n_fields = nf; a.inverse = 1;
-    PF_TRY(col<T>(p, N1, a, s));
+    a.n_fields = nf;
+    PF_TRY(col<T>(p, N1, K_INV, a, s));
     RowArgs r{};
     r.in = p->w1; r.out = out;
     r.n_rows = N0 * N1; r.N = N2; r.nh = (int)nh;
@@ -320,8 +341,8 @@ template <typename T> int backward_fields(pf_plan* p, const void* in, void* out,
     a.rows_in = N0; a.rows_out = N0; a.cols = (int)nh;
     a.in_row = nh; a.out_row = nh;
     a.in_field = p->nk; a.out_field = p->nk;
-    a.n_fields = nf; a.inverse = 1;
-    PF_TRY(col<T>(p, N0, a, s));
+    a.n_fields = nf;
+    PF_TRY(col<T>(p, N0, K_INV, a, s));
     RowArgs r{};
     r.in = p->w1; r.out = out;
     r.n_rows = N0; r.N = N1; r.nh = (int)nh;
@@ -350,15 +371,15 @@ template <typename T> int forward_fields(pf_plan* p, const void* in, void* out, 
     a.n_outer = N0; a.in_outer = (long long)N1 * nh; a.out_outer = a.in_outer;
     a.in_row = nh; a.out_row = nh;
     a.in_field = p->nk; a.out_field = p->nk;
-    a.n_fields = nf; a.inverse = 0; a.scale = 1.0 / N1;
-    PF_TRY(col<T>(p, N1, a, s));
+    a.n_fields = nf; a.scale = 1.0 / N1;
+    PF_TRY(col<T>(p, N1, K_FWD, a, s));
     a = col_args();
     a.in = out; a.out = out;
     a.rows_in = N0; a.rows_out = N0; a.cols = N1 * (int)nh;
     a.in_row = (long long)N1 * nh; a.out_row = a.in_row;
     a.in_field = p->nk; a.out_field = p->nk;
-    a.n_fields = nf; a.inverse = 0; a.scale = 1.0 / N0;
-    PF_TRY(col<T>(p, N0, a, s));
+    a.n_fields = nf; a.scale = 1.0 / N0;
+    PF_TRY(col<T>(p, N0, K_FWD, a, s));
   } else {
     const int N0 = p->N[0], N1 = p->N[1];
     RowArgs r{};
@@ -373,8 +394,8 @@ template <typename T> int forward_fields(pf_plan* p, const void* in, void* out, 
     a.rows_in = N0; a.rows_out = N0; a.cols = (int)nh;
     a.in_row = nh; a.out_row = nh;
     a.in_field = p->nk; a.out_field = p->nk;
-    a.n_fields = nf; a.inverse = 0; a.scale = 1.0 / N0;
-    PF_TRY(col<T>(p, N0, a, s));
+    a.n_fields = nf; a.scale = 1.0 / N0;
+    PF_TRY(col<T>(p, N0, K_FWD, a, s));
   }
   return 0;
 }
@@ -458,19 +479,31 @@ template <typename T> int step_impl(pf_plan* p, void* uhat, long long n_steps, c
   const int ns = n_stages(p->d.integrator);
   const bool noise_on = p->d.noise_mag != 0.0 && uses_noise(p->d.integrator);
   const long long noise_block = (long long)p->B * 2 * p->nc * p->nk;  // complex per step
+  const bool fused = p->fuse_stage && col_stage_width<T>(p->M[0]) > 0;
   for (long long st = 0; st < n_steps; ++st) {
     const void* nz = noise ? (const char*)noise + (size_t)(st * noise_block) * p->csz : nullptr;
     for (int stage = 0; stage < ns; ++stage) {
-      if (p->d.nonlinear) {
-        int rc = nonlinear_unprojected<T>(p, uhat, s);
-        if (rc) return rc;
-      }
       if (p->d.forcing == PF_FORCING_CONSTANT_POWER) {
         int rc = band_energy<T>(p, uhat, s);
         if (rc) return rc;
       }
-      const StageArgs a = stage_args(p, uhat, stage, nz, noise_on);
-      { PassTimer t(p, PF_PASS_STAGE, s); PF_TRY(launch_stage<T>(p->dim, a, s)); }
+      StageArgs a = stage_args(p, uhat, stage, nz, noise_on);
+      if (p->d.nonlinear && fused) {
+        int rc = nonlinear_front<T>(p, uhat, s);
+        if (rc) return rc;
+        ColArgs c = last_forward_args(p);
+        a.rhs_nl = nullptr;
+        c.st = a;
+        PassTimer t(p, PF_PASS_FWD0_STAGE, s);
+        PF_TRY(col<T>(p, p->M[0], p->dim == 3 ? K_FWD_STAGE3 : K_FWD_STAGE2, c, s));
+      } else {
+        if (p->d.nonlinear) {
+          int rc = nonlinear_unprojected<T>(p, uhat, s);
+          if (rc) return rc;
+        }
+        PassTimer t(p, PF_PASS_STAGE, s);
+        PF_TRY(launch_stage<T>(p->dim, a, s));
+      }
     }
     p->step += 1;
   }
@@ -754,6 +787,12 @@ int pf_profile_enable(pf_plan* p, int32_t enable) {
     v.clear();
   }
   p->prof = enable != 0;
+  return PF_OK;
+}
+
+int pf_set_fusion(pf_plan* p, int32_t fuse_stage) {
+  if (!p) return fail(PF_EINVAL, "null plan");
+  p->fuse_stage = fuse_stage != 0;
   return PF_OK;
 }
 

@@ -18,142 +18,6 @@ namespace pf {
 
 constexpr int kThreads = 256;
 
-// ---------------------------------------------------------------- column pass
-enum ColMode { COL_PLAIN = 0, COL_CURL3 = 1, COL_CURL2 = 2 };
-
-struct ColArgs {
-  const void* in;
-  void* out;
-  int rows_in, rows_out;        // rows read / written per matrix
-  int cols;                     // contiguous columns per matrix
-  int n_tiles;                  // column tiles per matrix
-  int n_outer;                  // matrices per field
-  long long in_row, out_row;    // element strides between rows
-  long long in_outer, out_outer;
-  long long in_field, out_field;
-  long long in_member, out_member;
-  int n_fields;                 // output fields per member
-  int n_members;
-  int inverse;                  // 1: zero-pad rows_in -> L, exp(+i..); 0: exp(-i..), truncate L -> rows_out
-  double scale;
-  int mode;
-  int gN0, gN1, gN2, gnh;       // unpadded grid (curl modes)
-  double ks0, ks1, ks2;         // wavenumber scales 2 pi / L_i
-  int mask23;                   // 2/3 rule mask while loading (curl modes)
-};
-
-template <typename T> constexpr int col_tc(int L) {
-  return sizeof(T) == 4 ? (L <= 128 ? 32 : (L <= 384 ? 16 : 8)) : (L <= 128 ? 16 : (L <= 384 ? 8 : 4));
-}
-
-// Row index of the padded spectrum -> source row and factor (3/2 rule,
-// complex axis).  The Nyquist row n/2 is split half-and-half between +-n/2.
-__device__ __forceinline__ int pad_src(int e, int n, int L, double& fac) {
-  fac = 1.0;
-  if (n == L) return e;
-  const int h = n / 2;
-  if (e < h) return e;
-  if (e > L - h) return e - (L - n);
-  if (e == h || e == L - h) { fac = 0.5; return h; }
-  return -1;
-}
-
-template <typename T, int L, int TC>
-__global__ void __launch_bounds__(kThreads) col_pass(ColArgs a, const cplx<T>* __restrict__ tw) {
-  constexpr int TCP = TC;
-  __shared__ cplx<T> tws[L];
-  __shared__ cplx<T> buf[L * TCP];
-  for (int m = threadIdx.x; m < L; m += kThreads) tws[m] = tw[m];
-
-  const int tile = blockIdx.x % a.n_tiles;
-  const int outer = blockIdx.x / a.n_tiles;
-  const int f = blockIdx.y % a.n_fields;
-  const int b = blockIdx.y / a.n_fields;
-  const int c0 = tile * TC;
-  const cplx<T>* in = (const cplx<T>*)a.in + (long long)b * a.in_member + (long long)outer * a.in_outer;
-  cplx<T>* out = (cplx<T>*)a.out + (long long)b * a.out_member + (long long)f * a.out_field +
-                 (long long)outer * a.out_outer;
-
-  for (int idx = threadIdx.x; idx < L * TC; idx += kThreads) {
-    const int e = idx / TC, c = idx % TC;
-    const int col = c0 + c;
-    cplx<T> v = mk<T>(0, 0);
-    if (col < a.cols) {
-      int src;
-      double fac = 1.0;
-      if (a.inverse) src = pad_src(e, a.rows_in, L, fac);
-      else src = e;
-      if (src >= 0) {
-        if (a.mode == COL_PLAIN) {
-          v = in[(long long)f * a.in_field + (long long)src * a.in_row + col];
-        } else if (a.mode == COL_CURL3) {
-          const int j = col / a.gnh, l = col % a.gnh;
-          const int mi = cmode(src, a.gN0), mj = cmode(j, a.gN1);
-          bool keep = true;
-          if (a.mask23) keep = abs(mi) <= a.gN0 / 3 && abs(mj) <= a.gN1 / 3 && l <= a.gN2 / 3;
-          if (keep) {
-            const cplx<T>* p = in + (long long)src * a.in_row + col;
-            if (f < 3) {
-              v = p[(long long)f * a.in_field];
-            } else {
-              const T k0 = (T)(mi * a.ks0), k1 = (T)(mj * a.ks1), k2 = (T)(l * a.ks2);
-              const cplx<T> u0 = p[0], u1 = p[a.in_field], u2 = p[2 * a.in_field];
-              cplx<T> w;
-              if (f == 3) w = mk<T>(k1 * u2.x - k2 * u1.x, k1 * u2.y - k2 * u1.y);
-              else if (f == 4) w = mk<T>(k2 * u0.x - k0 * u2.x, k2 * u0.y - k0 * u2.y);
-              else w = mk<T>(k0 * u1.x - k1 * u0.x, k0 * u1.y - k1 * u0.y);
-              v = mul_i<T>(w);
-            }
-          }
-        } else {  // COL_CURL2
-          const int l = col;
-          const int mi = cmode(src, a.gN0);
-          bool keep = true;
-          if (a.mask23) keep = abs(mi) <= a.gN0 / 3 && l <= a.gN1 / 3;
-          if (keep) {
-            const cplx<T>* p = in + (long long)src * a.in_row + col;
-            if (f < 2) {
-              v = p[(long long)f * a.in_field];
-            } else {
-              const T k0 = (T)(mi * a.ks0), k1 = (T)(l * a.ks1);
-              const cplx<T> u0 = p[0], u1 = p[a.in_field];
-              v = mul_i<T>(mk<T>(k0 * u1.x - k1 * u0.x, k0 * u1.y - k1 * u0.y));
-            }
-          }
-        }
-        if (fac != 1.0) v = scl<T>(v, (T)fac);
-      }
-    }
-    buf[e * TCP + c] = v;
-  }
-  __syncthreads();
-
-  auto addr = [](int ff, int e) { return e * TCP + ff; };
-  if (a.inverse) lds_fft<T, L, +1, TC, kThreads, true>(buf, tws, addr);
-  else lds_fft<T, L, -1, TC, kThreads, true>(buf, tws, addr);
-
-  const T s = (T)a.scale;
-  if (a.inverse) {
-    for (int idx = threadIdx.x; idx < L * TC; idx += kThreads) {
-      const int e = idx / TC, c = idx % TC;
-      const int col = c0 + c;
-      if (col < a.cols) out[(long long)e * a.out_row + col] = scl<T>(buf[e * TCP + c], s);
-    }
-  } else {
-    const int n = a.rows_out, h = n / 2;
-    for (int idx = threadIdx.x; idx < n * TC; idx += kThreads) {
-      const int o = idx / TC, c = idx % TC;
-      const int col = c0 + c;
-      if (col >= a.cols) continue;
-      cplx<T> v;
-      if (n == L || o < h) v = buf[o * TCP + c];
-      else if (o == h) v = add<T>(buf[h * TCP + c], buf[(L - h) * TCP + c]);
-      else v = buf[(o + (L - n)) * TCP + c];
-      out[(long long)o * a.out_row + col] = scl<T>(v, s);
-    }
-  }
-}
-
 // ------------------------------------------------------------------- row pass
 enum RowMode { ROW_FUSED3 = 0, ROW_FUSED2 = 1, ROW_C2R = 2, ROW_R2C = 3 };
 
@@ -222,53 +86,90 @@ __global__ void __launch_bounds__(kThreads) row_pass(RowArgs a, const cplx<T>* _
   auto ix = [](int slot, int e) { return slot * LP + e + (e >> 4); };
   const int N = a.N;
 
-  // ---- load
+  // ---- load (every load of a chunk in flight before the LDS stores)
   if constexpr (MODE == ROW_R2C) {
     const T* in = (const T*)a.in + (long long)b * a.in_member + (long long)f * a.in_field;
-    for (int t = threadIdx.x; t < NP * L; t += kThreads) {
-      const int p = t / L, x = t % L;
-      const int pr = pair0 + p;
-      cplx<T> z = mk<T>(0, 0);
-      if (pr < npairs) {
-        const long long r0 = 2LL * pr;
-        z = mk<T>(in[r0 * a.in_row + x], in[(r0 + 1) * a.in_row + x]);
+    constexpr int NIT = (NP * L + kThreads - 1) / kThreads;
+    constexpr int CH = NIT < 8 ? NIT : 8;
+    for (int base = 0; base < NIT; base += CH) {
+      T xr[CH], yr[CH];
+#pragma unroll
+      for (int q = 0; q < CH; ++q) {
+        const int t = threadIdx.x + (base + q) * kThreads;
+        xr[q] = yr[q] = (T)0;
+        if (base + q >= NIT || t >= NP * L) continue;
+        const int p = t / L, x = t % L;
+        const int pr = pair0 + p;
+        if (pr < npairs) {
+          const long long r0 = 2LL * pr;
+          xr[q] = in[r0 * a.in_row + x];
+          yr[q] = in[(r0 + 1) * a.in_row + x];
+        }
       }
-      buf[ix(p, x)] = z;
+#pragma unroll
+      for (int q = 0; q < CH; ++q) {
+        const int t = threadIdx.x + (base + q) * kThreads;
+        if (base + q >= NIT || t >= NP * L) continue;
+        buf[ix(t / L, t % L)] = mk<T>(xr[q], yr[q]);
+      }
     }
   } else {
+    // stored entries kk = 0..L/2 of each packed pair (X, Y); Z_kk = X + iY and
+    // Z_{L-kk} = conj(X) + i conj(Y) (Hermitian extension)
     const cplx<T>* in = (const cplx<T>*)a.in + (long long)b * a.in_member;
-    for (int t = threadIdx.x; t < NP * NS * L; t += kThreads) {
-      const int p = t / (NS * L), rem = t % (NS * L);
-      const int s = rem / L, k = rem % L;
-      const int pr = pair0 + p;
-      cplx<T> z = mk<T>(0, 0);
-      if (pr < npairs) {
+    constexpr int HK = L / 2 + 1;
+    constexpr int NLD = NP * NS * HK;
+    constexpr int NIT = (NLD + kThreads - 1) / kThreads;
+    constexpr int CH = NIT < 8 ? NIT : 8;
+    const int kmax = a.mask23 ? N / 3 : N / 2;
+    for (int base = 0; base < NIT; base += CH) {
+      cplx<T> X[CH], Y[CH];
+#pragma unroll
+      for (int q = 0; q < CH; ++q) {
+        const int t = threadIdx.x + (base + q) * kThreads;
+        X[q] = Y[q] = mk<T>(0, 0);
+        if (base + q >= NIT || t >= NLD) continue;
+        const int p = t / (NS * HK), rem = t % (NS * HK);
+        const int sl = rem / HK, kk = rem % HK;
+        const int pr = pair0 + p;
+        if (pr >= npairs || kk > kmax) continue;
         const long long r0 = 2LL * pr, r1 = r0 + 1;
-        cplx<T> X, Y;
+        const cplx<T>* px;
+        const cplx<T>* py;
         if constexpr (MODE == ROW_FUSED3) {
-          const int rr = s / 3, q = s % 3;
+          const int rr = sl / 3, qq = sl % 3;
           const long long r = r0 + rr;
-          X = hext<T>(in + (2 * q) * a.in_field + r * a.in_row, k, L, N, a.mask23);
-          Y = hext<T>(in + (2 * q + 1) * a.in_field + r * a.in_row, k, L, N, a.mask23);
+          px = in + (2 * qq) * a.in_field + r * a.in_row;
+          py = in + (2 * qq + 1) * a.in_field + r * a.in_row;
         } else if constexpr (MODE == ROW_FUSED2) {
-          if (s == 0) {
-            X = hext<T>(in + 0 * a.in_field + r0 * a.in_row, k, L, N, a.mask23);
-            Y = hext<T>(in + 1 * a.in_field + r0 * a.in_row, k, L, N, a.mask23);
-          } else if (s == 1) {
-            X = hext<T>(in + 2 * a.in_field + r0 * a.in_row, k, L, N, a.mask23);
-            Y = hext<T>(in + 2 * a.in_field + r1 * a.in_row, k, L, N, a.mask23);
-          } else {
-            X = hext<T>(in + 0 * a.in_field + r1 * a.in_row, k, L, N, a.mask23);
-            Y = hext<T>(in + 1 * a.in_field + r1 * a.in_row, k, L, N, a.mask23);
-          }
+          if (sl == 0) { px = in + r0 * a.in_row; py = in + a.in_field + r0 * a.in_row; }
+          else if (sl == 1) { px = in + 2 * a.in_field + r0 * a.in_row; py = in + 2 * a.in_field + r1 * a.in_row; }
+          else { px = in + r1 * a.in_row; py = in + a.in_field + r1 * a.in_row; }
         } else {  // ROW_C2R
           const cplx<T>* fin = in + (long long)f * a.in_field;
-          X = hext<T>(fin + r0 * a.in_row, k, L, N, a.mask23);
-          Y = hext<T>(fin + r1 * a.in_row, k, L, N, a.mask23);
+          px = fin + r0 * a.in_row;
+          py = fin + r1 * a.in_row;
         }
-        z = pack<T>(X, Y);
+        X[q] = px[kk];
+        Y[q] = py[kk];
       }
-      buf[ix(p * NS + s, k)] = z;
+#pragma unroll
+      for (int q = 0; q < CH; ++q) {
+        const int t = threadIdx.x + (base + q) * kThreads;
+        if (base + q >= NIT || t >= NLD) continue;
+        const int p = t / (NS * HK), rem = t % (NS * HK);
+        const int sl = rem / HK, kk = rem % HK;
+        cplx<T> x = X[q], y = Y[q];
+        if (kk == 0) { x.y = 0; y.y = 0; }
+        if (kk == N / 2) {
+          x.y = 0;
+          y.y = 0;
+          if (L != N) { x.x *= (T)0.5; y.x *= (T)0.5; }
+        }
+        const int slot = p * NS + sl;
+        buf[ix(slot, kk)] = pack<T>(x, y);
+        if (kk > 0 && kk < L / 2) buf[ix(slot, L - kk)] = pack<T>(conj<T>(x), conj<T>(y));
+      }
     }
   }
   __syncthreads();
@@ -510,11 +411,13 @@ template <typename T, int D> __device__ __forceinline__ void project(cplx<T>* v,
   for (int i = 0; i < D; ++i) v[i] = sub<T>(v[i], scl<T>(p, m.k[i]));
 }
 
+// One mode of one member: assemble the stage's terms and update u_hat.
+// nlin: the unprojected nonlinear term of this mode when the caller already
+// holds it (fused forward pass), else it is read from a.rhs_nl.
+// upre / uzpre: u_hat and u0 of this mode already fetched (stage_fetch).
 template <typename T, int D>
-__global__ void __launch_bounds__(kThreads) stage_kernel(StageArgs a) {
-  const long long idx = (long long)blockIdx.x * kThreads + threadIdx.x;
-  const int b = blockIdx.y;
-  if (idx >= a.nk) return;
+__device__ __forceinline__ void stage_point(const StageArgs& a, long long idx, int b, const cplx<T>* nlin,
+                                            const cplx<T>* upre = nullptr, const cplx<T>* uzpre = nullptr) {
   const Mode<T, D> m = decode<T, D>(a, idx);
   const long long mb = (long long)b * D * a.nk;
   cplx<T>* uh = (cplx<T>*)a.uhat + mb + idx;
@@ -523,8 +426,8 @@ __global__ void __launch_bounds__(kThreads) stage_kernel(StageArgs a) {
   cplx<T> u[D], uz[D], nl[D], nz[D], fr[D];
 #pragma unroll
   for (int i = 0; i < D; ++i) {
-    u[i] = uh[i * a.nk];
-    uz[i] = (a.save_u0 || a.mode == 1) ? u[i] : u0p[i * a.nk];
+    u[i] = upre ? upre[i] : uh[i * a.nk];
+    uz[i] = (a.save_u0 || a.mode == 1) ? u[i] : (uzpre ? uzpre[i] : u0p[i * a.nk]);
     nl[i] = mk<T>(0, 0);
     nz[i] = mk<T>(0, 0);
     fr[i] = mk<T>(0, 0);
@@ -578,10 +481,15 @@ __global__ void __launch_bounds__(kThreads) stage_kernel(StageArgs a) {
   }
 
   // nonlinear term: Leray projection (math.py:72-100) and optional Nyquist mask
-  if (a.rhs_nl) {
-    const cplx<T>* r = (const cplx<T>*)a.rhs_nl + mb + idx;
+  if (nlin || a.rhs_nl) {
+    if (nlin) {
 #pragma unroll
-    for (int i = 0; i < D; ++i) nl[i] = r[i * a.nk];
+      for (int i = 0; i < D; ++i) nl[i] = nlin[i];
+    } else {
+      const cplx<T>* r = (const cplx<T>*)a.rhs_nl + mb + idx;
+#pragma unroll
+      for (int i = 0; i < D; ++i) nl[i] = r[i * a.nk];
+    }
     project<T, D>(nl, m);
     if (a.mask_nyq && m.nyq) {
 #pragma unroll
@@ -686,6 +594,24 @@ __global__ void __launch_bounds__(kThreads) stage_kernel(StageArgs a) {
     }
     uh[i * a.nk] = r;
   }
+}
+
+// Loads of stage_point hoisted so that several modes' loads are in flight.
+template <typename T, int D>
+__device__ __forceinline__ void stage_fetch(const StageArgs& a, long long idx, int b, cplx<T>* u, cplx<T>* uz) {
+  const long long mb = (long long)b * D * a.nk + idx;
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    u[i] = ((const cplx<T>*)a.uhat)[mb + i * a.nk];
+    uz[i] = a.save_u0 ? mk<T>(0, 0) : ((const cplx<T>*)a.u0)[mb + i * a.nk];
+  }
+}
+
+template <typename T, int D>
+__global__ void __launch_bounds__(kThreads) stage_kernel(StageArgs a) {
+  const long long idx = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (idx >= a.nk) return;
+  stage_point<T, D>(a, idx, blockIdx.y, nullptr);
 }
 
 // ------------------------------------------------------------- reductions (FP64)
@@ -851,6 +777,250 @@ __global__ void __launch_bounds__(kThreads) spectrum_kernel(DiagArgs a, int nshe
     int sh = (int)rint(sqrt(m.k2));
     if (sh > nshell - 1) sh = nshell - 1;
     atomicAdd(&a.out[(long long)b * nshell + sh], 0.5 * w * s);
+  }
+}
+
+
+// ---------------------------------------------------------------- column pass
+// FFT along a strided axis: a block loads a tile of L rows x TC contiguous
+// columns into LDS (coalesced row segments), transforms every column and
+// writes the tile back.  Kinds:
+//   K_INV        inverse, rows_in -> L with 3/2 zero padding (Nyquist split)
+//   K_INV_CURL3  axis 0 of 3D: emits u_hat and omega_hat = i k x u_hat
+//   K_INV_CURL2  axis 0 of 2D: emits u_hat and the scalar vorticity
+//   K_FWD        forward, L -> rows_out with truncation (Nyquist sum)
+//   K_FWD_STAGE3/2  forward axis 0 of all components of a member, then the
+//                stage update of every mode of the tile (no R buffer)
+// Under the 2/3 rule ("prune") the inverse passes visit only retained
+// modes: rows with |k| > Krow are zero without a load, and the axis-0 pass
+// of 3D enumerates only the (j, l) columns with |k_j| <= K1, l <= K2; the
+// columns it skips are never read downstream.
+enum ColKind { K_INV = 0, K_INV_CURL3 = 1, K_INV_CURL2 = 2, K_FWD = 3, K_FWD_STAGE3 = 4, K_FWD_STAGE2 = 5 };
+
+struct ColArgs {
+  const void* in;
+  void* out;
+  int rows_in, rows_out;        // rows read / written per matrix
+  int cols;                     // columns per matrix (compact count when pruned)
+  int n_tiles;                  // column tiles per matrix (set by the launcher)
+  int n_outer;                  // matrices per field
+  long long in_row, out_row;    // element strides between rows
+  long long in_outer, out_outer;
+  long long in_field, out_field;
+  long long in_member, out_member;
+  int n_fields;                 // fields per member
+  int n_members;
+  double scale;
+  int gN0, gN1, gN2, gnh;       // unpadded grid (curl kinds)
+  double ks0, ks1, ks2;         // wavenumber scales 2 pi / L_i
+  int mask23;                   // 2/3 mask while loading (curl kinds)
+  int prune;                    // visit retained modes only (2/3 rule)
+  int Krow;                     // pruned inverse: rows with |k| > Krow are zero
+  int K1, n2k;                  // pruned 3D axis-0: |k_j| <= K1, l < n2k
+  StageArgs st;                 // fused stage kinds
+};
+
+template <typename T> constexpr int col_tc(int L) {
+  return sizeof(T) == 4 ? (L <= 128 ? 32 : (L <= 384 ? 16 : 8)) : (L <= 128 ? 16 : (L <= 384 ? 8 : 4));
+}
+// tile width of the fused forward + stage kinds (0: not available)
+template <typename T> constexpr int col_tc_stage(int L) {
+  return sizeof(T) == 4 ? (L <= 384 ? 8 : (L <= 768 ? 4 : 0)) : (L <= 384 ? 4 : (L <= 768 ? 2 : 0));
+}
+template <typename T, int KIND> constexpr int col_width(int L) {
+  return (KIND == K_FWD_STAGE3 || KIND == K_FWD_STAGE2) ? col_tc_stage<T>(L) : col_tc<T>(L);
+}
+
+// Row index of the padded spectrum -> source row and factor (3/2 rule,
+// complex axis).  The Nyquist row n/2 is split half-and-half between +-n/2.
+__device__ __forceinline__ int pad_src(int e, int n, int L, double& fac) {
+  fac = 1.0;
+  if (n == L) return e;
+  const int h = n / 2;
+  if (e < h) return e;
+  if (e > L - h) return e - (L - n);
+  if (e == h || e == L - h) { fac = 0.5; return h; }
+  return -1;
+}
+
+// Consecutive block ids are dealt round-robin to the 8 XCDs; give each XCD a
+// contiguous run of logical tiles so neighbours share its L2.
+__device__ __forceinline__ long long xcd_order(long long bid, long long G) {
+  const long long q = G / 8, r = G % 8, x = bid % 8;
+  return x * q + (x < r ? x : r) + bid / 8;
+}
+
+template <typename T, int L, int TC, int KIND>
+__global__ void __launch_bounds__(kThreads) col_pass(ColArgs a, const cplx<T>* __restrict__ tw) {
+  constexpr bool STAGE = KIND == K_FWD_STAGE3 || KIND == K_FWD_STAGE2;
+  constexpr bool INV = KIND == K_INV || KIND == K_INV_CURL3 || KIND == K_INV_CURL2;
+  constexpr bool CURL = KIND == K_INV_CURL3 || KIND == K_INV_CURL2;
+  constexpr int NFLD = KIND == K_FWD_STAGE3 ? 3 : (KIND == K_FWD_STAGE2 ? 2 : 1);
+  constexpr int W = NFLD * TC;  // LDS columns
+  constexpr int NIT = (L * W + kThreads - 1) / kThreads;
+  constexpr int CH = NIT < 16 ? NIT : 16;  // loads kept in flight per thread
+  __shared__ cplx<T> tws[L];
+  __shared__ cplx<T> buf[L * W];
+  for (int m = threadIdx.x; m < L; m += kThreads) tws[m] = tw[m];
+
+  const int nfg = STAGE ? 1 : a.n_fields;
+  long long lg = xcd_order(blockIdx.x, gridDim.x);
+  const int fg = (int)(lg % nfg);
+  lg /= nfg;
+  const int tile = (int)(lg % a.n_tiles);
+  lg /= a.n_tiles;
+  const int outer = (int)(lg % a.n_outer);
+  const int b = (int)(lg / a.n_outer);
+  const int c0 = tile * TC;
+  const cplx<T>* in = (const cplx<T>*)a.in + (long long)b * a.in_member + (long long)outer * a.in_outer;
+
+  // compact column -> column of the matrix
+  auto colmap = [&](int cc) -> int {
+    if (KIND == K_INV_CURL3 && a.prune) {
+      const int jj = cc / a.n2k, l = cc % a.n2k;
+      const int j = jj <= a.K1 ? jj : jj + a.gN1 - (2 * a.K1 + 1);
+      return j * a.gnh + l;
+    }
+    return cc;
+  };
+
+  // ---- load: per chunk, issue every load, then combine and store to LDS
+  for (int base = 0; base < NIT; base += CH) {
+    cplx<T> ra[CH], rb[CH];
+    T ka[CH], kb[CH], fac[CH];
+    bool ok[CH];
+#pragma unroll
+    for (int q = 0; q < CH; ++q) {
+      const int idx = threadIdx.x + (base + q) * kThreads;
+      ra[q] = mk<T>(0, 0);
+      rb[q] = mk<T>(0, 0);
+      ka[q] = kb[q] = (T)0;
+      fac[q] = (T)1;
+      ok[q] = false;
+      if (base + q >= NIT || idx >= L * W) continue;
+      const int e = idx / W, cf = idx % W;
+      const int f = STAGE ? cf / TC : fg;
+      const int c = STAGE ? cf % TC : cf;
+      const int cc = c0 + c;
+      if (cc >= a.cols) continue;
+      const int col = colmap(cc);
+      int src = e;
+      double fc = 1.0;
+      if (INV) {
+        src = pad_src(e, a.rows_in, L, fc);
+        if (KIND == K_INV && a.prune && src >= 0 && abs(cmode(src, a.rows_in)) > a.Krow) src = -1;
+      }
+      if (src < 0) continue;
+      fac[q] = (T)fc;
+      if constexpr (CURL) {
+        constexpr int D = KIND == K_INV_CURL3 ? 3 : 2;
+        int mi = cmode(src, a.gN0), mj = 0, l;
+        if (D == 3) { mj = cmode(col / a.gnh, a.gN1); l = col % a.gnh; }
+        else l = col;
+        if (a.mask23 && (abs(mi) > a.gN0 / 3 || abs(mj) > a.gN1 / 3 || l > (D == 3 ? a.gN2 : a.gN1) / 3))
+          continue;
+        const cplx<T>* p = in + (long long)src * a.in_row + col;
+        const T k0 = (T)(mi * a.ks0);
+        const T k1 = D == 3 ? (T)(mj * a.ks1) : (T)(l * a.ks1);
+        const T k2 = D == 3 ? (T)(l * a.ks2) : (T)0;
+        ok[q] = true;
+        if (f < D) {
+          ra[q] = p[(long long)f * a.in_field];
+          ka[q] = (T)1;  // marks a plain copy
+          kb[q] = (T)0;
+          continue;
+        }
+        // omega_f = i (ka * u_ia - kb * u_ib)
+        int ia, ib;
+        if (D == 2) { ia = 1; ib = 0; ka[q] = k0; kb[q] = k1; }
+        else if (f == 3) { ia = 2; ib = 1; ka[q] = k1; kb[q] = k2; }
+        else if (f == 4) { ia = 0; ib = 2; ka[q] = k2; kb[q] = k0; }
+        else { ia = 1; ib = 0; ka[q] = k0; kb[q] = k1; }
+        ra[q] = p[(long long)ia * a.in_field];
+        rb[q] = p[(long long)ib * a.in_field];
+      } else {
+        ok[q] = true;
+        ra[q] = in[(long long)f * a.in_field + (long long)src * a.in_row + col];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < CH; ++q) {
+      const int idx = threadIdx.x + (base + q) * kThreads;
+      if (base + q >= NIT || idx >= L * W) continue;
+      cplx<T> v = mk<T>(0, 0);
+      if (ok[q]) {
+        if constexpr (CURL) {
+          const int f = fg;
+          if (f < (KIND == K_INV_CURL3 ? 3 : 2)) v = ra[q];
+          else v = mul_i<T>(mk<T>(ka[q] * ra[q].x - kb[q] * rb[q].x, ka[q] * ra[q].y - kb[q] * rb[q].y));
+        } else {
+          v = ra[q];
+        }
+        v = scl<T>(v, fac[q]);
+      }
+      buf[idx] = v;  // idx = e * W + cf
+    }
+  }
+  __syncthreads();
+
+  auto addr = [](int ff, int e) { return e * W + ff; };
+  if constexpr (INV) lds_fft<T, L, +1, W, kThreads, true>(buf, tws, addr);
+  else lds_fft<T, L, -1, W, kThreads, true>(buf, tws, addr);
+
+  const T s = (T)a.scale;
+  if constexpr (INV) {
+    cplx<T>* out = (cplx<T>*)a.out + (long long)b * a.out_member + (long long)fg * a.out_field +
+                   (long long)outer * a.out_outer;
+    constexpr int NO = (L * TC + kThreads - 1) / kThreads;
+#pragma unroll
+    for (int q = 0; q < NO; ++q) {
+      const int idx = threadIdx.x + q * kThreads;
+      if (idx >= L * TC) continue;
+      const int e = idx / TC, c = idx % TC;
+      const int cc = c0 + c;
+      if (cc < a.cols) out[(long long)e * a.out_row + colmap(cc)] = scl<T>(buf[e * W + c], s);
+    }
+  } else {
+    const int n = a.rows_out, h = n / 2;
+    auto trunc = [&](int o, int cf) -> cplx<T> {
+      cplx<T> v;
+      if (n == L || o < h) v = buf[o * W + cf];
+      else if (o == h) v = add<T>(buf[h * W + cf], buf[(L - h) * W + cf]);
+      else v = buf[(o + (L - n)) * W + cf];
+      return scl<T>(v, s);
+    };
+    if constexpr (STAGE) {
+      // modes of the tile: fetch u_hat and u0 of SCH modes, then update them
+      constexpr int SCH = 4;
+      for (int idx0 = threadIdx.x; idx0 < n * TC; idx0 += SCH * kThreads) {
+        cplx<T> uu[SCH][NFLD], zz[SCH][NFLD];
+#pragma unroll
+        for (int q = 0; q < SCH; ++q) {
+          const int idx = idx0 + q * kThreads;
+          const int o = idx / TC, cc = c0 + idx % TC;
+          if (idx < n * TC && cc < a.cols) stage_fetch<T, NFLD>(a.st, (long long)o * a.cols + cc, b, uu[q], zz[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < SCH; ++q) {
+          const int idx = idx0 + q * kThreads;
+          const int o = idx / TC, c = idx % TC;
+          const int cc = c0 + c;
+          if (idx >= n * TC || cc >= a.cols) continue;
+          cplx<T> nl[NFLD];
+#pragma unroll
+          for (int f = 0; f < NFLD; ++f) nl[f] = trunc(o, f * TC + c);
+          stage_point<T, NFLD>(a.st, (long long)o * a.cols + cc, b, nl, uu[q], zz[q]);
+        }
+      }
+    } else {
+      cplx<T>* out = (cplx<T>*)a.out + (long long)b * a.out_member + (long long)fg * a.out_field +
+                     (long long)outer * a.out_outer;
+      for (int idx = threadIdx.x; idx < n * TC; idx += kThreads) {
+        const int o = idx / TC, c = idx % TC;
+        const int cc = c0 + c;
+        if (cc < a.cols) out[(long long)o * a.out_row + cc] = trunc(o, c);
+      }
+    }
   }
 }
 

@@ -12,8 +12,10 @@ namespace pf {
 
 bool length_supported(int L);
 
-// Launch one column pass (FFT along a strided axis).  Fills a.n_tiles.
-template <typename T> int launch_col(int L, ColArgs a, const void* tw, hipStream_t s);
+// Launch one column pass (FFT along a strided axis) of the given ColKind.
+template <typename T> int launch_col(int L, int kind, const ColArgs& a, const void* tw, hipStream_t s);
+// Tile width of the fused forward + stage kinds at length L (0: unavailable).
+template <typename T> int col_stage_width(int L);
 // Launch one row pass (last axis) of the given RowMode.
 template <typename T> int launch_row(int L, int mode, const RowArgs& a, const void* tw, hipStream_t s);
 

@@ -18,7 +18,8 @@ PF_DEALIAS_PAD32, PF_DEALIAS_TRUNC23 = 0, 1
 INTEGRATOR_IDS = {"explicit_pc": 0, "implicit_pc": 1, "rk2": 2, "rk3": 3, "rk4": 4}
 PF_NOISE_THERMAL, PF_NOISE_CORRELATED = 0, 1
 PF_FORCING_NONE, PF_FORCING_CONSTANT_POWER = 0, 1
-PASS_NAMES = ("inv_axis0_curl", "inv_axis1", "row_c2r_cross_r2c", "fwd_axis1", "fwd_axis0", "stage", "band")
+PASS_NAMES = ("inv_axis0_curl", "inv_axis1", "row_c2r_cross_r2c", "fwd_axis1", "fwd_axis0", "stage", "band",
+              "fwd_axis0_stage")
 
 
 class PfDesc(C.Structure):
@@ -79,6 +80,7 @@ _SIGS = {
     "pf_accumulate_sk": (C.c_int, [_P, _P, _P, _P]),
     "pf_profile_enable": (C.c_int, [_P, C.c_int32]),
     "pf_profile_read": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32]),
+    "pf_set_fusion": (C.c_int, [_P, C.c_int32]),
     "pf_last_error": (C.c_char_p, []),
     "pf_version": (C.c_char_p, []),
 }

@@ -223,6 +223,10 @@ class Plan:
         call("pf_spectrum", self._h, _ptr(u_hat), _ptr(out), int(nshell), _stream())
         return out
 
+    def set_fusion(self, fuse_stage=True):
+        """Fuse the axis-0 forward pass with the stage update in step() (default on)."""
+        call("pf_set_fusion", self._h, int(bool(fuse_stage)))
+
     def profile(self, enable=True):
         """Start (or stop) per-pass HIP-event timing; clears earlier records."""
         call("pf_profile_enable", self._h, int(bool(enable)))

@@ -96,11 +96,15 @@ def test_null_arguments(lib):
 def test_perfmodel_matches_byte_table():
     from periodicflow import perfmodel
 
-    # 256^3, 2/3 rule, FP32: nk = 256*256*129 modes; stage kernel moves 12 nk complex
-    pb = perfmodel.pass_bytes([256] * 3, "2/3", "fp32")
+    # 256^3, 2/3 rule, FP32, unfused: the stage kernel moves 12 nk complex
     nk = 256 * 256 * 129
+    pb = perfmodel.pass_bytes([256] * 3, "2/3", "fp32", fused=False)
     assert pb["stage"] == 12 * nk * 8
-    assert pb["inv_axis1"] == 12 * nk * 8
-    # 3/2 rule pads the inverse outputs by 3/2 per padded axis
+    # pruned inverse passes: 171 retained rows per complex axis, 86 entries on the last
+    assert pb["inv_axis1"] == (6 * 256 * 171 * 86 + 6 * 256 * 256 * 86) * 8
+    fused = perfmodel.pass_bytes([256] * 3, "2/3", "fp32")
+    assert fused["fwd_axis0_stage"] == (3 * 256 * 256 * 129 + 9 * nk) * 8
+    assert "stage" not in fused
+    # 3/2 rule pads the inverse outputs by 3/2 per padded axis, nothing pruned
     pb32 = perfmodel.pass_bytes([64] * 3, "3/2", "fp64")
     assert pb32["inv_axis0_curl"] == (3 * 64 * 64 * 33 + 6 * 96 * 64 * 33) * 16

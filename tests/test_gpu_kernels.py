@@ -199,3 +199,21 @@ def test_forcing_parity_fp64(pf):
     ud = _dev(u_init[None], plan.complex_dtype)
     plan.step(ud, 3)
     assert po.rel_l2(_host(ud)[0], o.u_hat) < FP64_TOL
+
+
+@pytest.mark.parametrize("N,dealias,integ", [((8, 12, 16), "3/2", "rk3"), ((16, 16, 16), "2/3", "rk4"),
+                                             ((24, 16), "2/3", "implicit_pc"), ((16, 24), "3/2", "explicit_pc")])
+def test_fused_stage_equals_unfused(pf, N, dealias, integ):
+    _say("fused vs unfused", N, dealias, integ)
+    rs = np.random.RandomState(9)
+    shape = (2, len(N)) + po.Grid(N).shape_fourier
+    u = 0.2 * _rand_spec(rs, shape)
+    out = []
+    for fuse in (True, False):
+        plan = pf.Plan(N, 0.01, precision="fp64", integrator=integ, viscosity=0.02, dealias=dealias, batch=2,
+                       noise_mag=0.1, seeds=[3, 4])
+        plan.set_fusion(fuse)
+        ud = _dev(u, plan.complex_dtype)
+        plan.step(ud, 3)
+        out.append(_host(ud))
+    assert po.rel_l2(out[0], out[1]) < 1e-14
