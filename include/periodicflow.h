@@ -90,9 +90,21 @@ int pf_plan_get_info(const pf_plan* plan, pf_plan_info* info);
 
 /* Per-member RNG seeds (host array of `batch` values) for on-device noise. */
 int pf_set_seeds(pf_plan* plan, const uint64_t* seeds);
-/* Step counter that keys the on-device noise (starts at 0). */
+/* Step counter: pf_get_step_counter returns the number of steps pf_step
+ * has enqueued; pf_set_step_counter sets it and every member's device-side
+ * RNG step counter (starts at 0). */
 int pf_set_step_counter(pf_plan* plan, int64_t step);
 int64_t pf_get_step_counter(const pf_plan* plan);
+/* Slot management for continuous batching (no reference counterpart):
+ * the on-device noise of member m is keyed by (seed[m], its own step
+ * counter), which advances on the device once per step while the member is
+ * active, so a member's result does not depend on its neighbours or on
+ * when it joined.  pf_set_member sets one slot's seed and step counter;
+ * pf_set_active limits every call to members [0, n) (default: batch);
+ * pf_get_member_steps copies the batch counters to the host. */
+int pf_set_member(pf_plan* plan, int32_t member, uint64_t seed, int64_t step);
+int pf_set_active(pf_plan* plan, int32_t n);
+int pf_get_member_steps(pf_plan* plan, int64_t* out);
 
 /* L: device array (N0, [N1,] nh) of reals, shared by all members; NULL
  * restores -nu |k|^2.  Copied into the plan. */
@@ -118,7 +130,7 @@ int pf_rhs_nonlinear(pf_plan* plan, const void* u_hat, void* rhs, void* stream);
  * noise == NULL: noise drawn on the device (Philox4x32-10 keyed by the
  *   member seed, the step counter, the mode and the component).
  * noise != NULL: raw standard-normal draws, n_steps blocks of
- *   (batch, 2, noise_components, nk) complex values (w1 then w2), the
+ *   (active, 2, noise_components, nk) complex values (w1 then w2), the
  *   quantities np.random.randn fills in random_fields before scaling.
  * The step counter advances by n_steps. */
 int pf_step(pf_plan* plan, void* u_hat, int64_t n_steps, const void* noise, void* stream);

@@ -65,6 +65,8 @@ struct pf_plan {
   void* lin_op = nullptr;
   void* corr = nullptr;
   unsigned long long* seeds = nullptr;
+  long long* msteps = nullptr;   // per-member RNG step counters (device)
+  int nact = 1;                  // members [0, nact) are advanced
   double* fcoef = nullptr;
   std::map<int, void*> tw;
   long long step = 0;
@@ -156,7 +158,7 @@ int make_twiddles(pf_plan* p, int L) {
 StageArgs geometry(const pf_plan* p) {
   StageArgs g{};
   g.nk = p->nk;
-  g.n_members = p->B;
+  g.n_members = p->nact;
   g.nh = p->nh;
   g.N0 = p->N[0];
   g.N1 = p->N[1];
@@ -188,7 +190,7 @@ ColArgs col_args() {
 // of the nonlinear term: u_hat -> input of the axis-0 forward pass (w2),
 // laid out (B, d, M0, [N1,] nh).
 template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStream_t s) {
-  const int B = p->B;
+  const int B = p->nact;
   const long long nh = p->nh;
   const bool pr = p->direct23;
   if (p->dim == 3) {
@@ -293,7 +295,7 @@ ColArgs last_forward_args(const pf_plan* p) {
   a.in_row = inner; a.out_row = inner;
   a.in_field = D_f; a.out_field = p->nk;
   a.in_member = d * D_f; a.out_member = d * p->nk;
-  a.n_fields = d; a.n_members = p->B; a.scale = 1.0 / M0;
+  a.n_fields = d; a.n_members = p->nact; a.scale = 1.0 / M0;
   return a;
 }
 
@@ -437,7 +439,7 @@ template <typename T> int band_energy(pf_plan* p, const void* uhat, hipStream_t 
   ba.kmax2 = p->d.forcing_kmax * p->d.forcing_kmax;
   ba.kmax = p->d.forcing_kmax;
   ba.power = p->d.forcing_power;
-  { PassTimer t(p, PF_PASS_BAND, s); PF_TRY(launch_band<T>(p->dim, ba, p->B, s)); }
+  { PassTimer t(p, PF_PASS_BAND, s); PF_TRY(launch_band<T>(p->dim, ba, p->nact, s)); }
   return 0;
 }
 
@@ -471,6 +473,7 @@ StageArgs stage_args(const pf_plan* p, void* uhat, int stage, const void* noise,
   a.fk2min = p->d.forcing_kmin * p->d.forcing_kmin;
   a.fk2max = p->d.forcing_kmax * p->d.forcing_kmax;
   a.step = p->step;
+  a.msteps = p->msteps;
   a.mode = 0;
   return a;
 }
@@ -478,7 +481,7 @@ StageArgs stage_args(const pf_plan* p, void* uhat, int stage, const void* noise,
 template <typename T> int step_impl(pf_plan* p, void* uhat, long long n_steps, const void* noise, hipStream_t s) {
   const int ns = n_stages(p->d.integrator);
   const bool noise_on = p->d.noise_mag != 0.0 && uses_noise(p->d.integrator);
-  const long long noise_block = (long long)p->B * 2 * p->nc * p->nk;  // complex per step
+  const long long noise_block = (long long)p->nact * 2 * p->nc * p->nk;  // complex per step
   const bool fused = p->fuse_stage && col_stage_width<T>(p->M[0]) > 0;
   for (long long st = 0; st < n_steps; ++st) {
     const void* nz = noise ? (const char*)noise + (size_t)(st * noise_block) * p->csz : nullptr;
@@ -505,6 +508,7 @@ template <typename T> int step_impl(pf_plan* p, void* uhat, long long n_steps, c
         PF_TRY(launch_stage<T>(p->dim, a, s));
       }
     }
+    PF_TRY(launch_member_step(p->msteps, p->nact, s));
     p->step += 1;
   }
   return 0;
@@ -602,6 +606,7 @@ int pf_plan_create(const pf_desc* desc, pf_plan** out) {
       (w2c && (rc = dev_alloc(p, &p->w2, p->w2_bytes))) ||
       (rc = dev_alloc(p, &p->u0, (size_t)B * d.dim * p->nk * p->csz)) ||
       (rc = dev_alloc(p, (void**)&p->seeds, (size_t)B * sizeof(unsigned long long))) ||
+      (rc = dev_alloc(p, (void**)&p->msteps, (size_t)B * sizeof(long long))) ||
       (rc = dev_alloc(p, (void**)&p->fcoef, (size_t)B * sizeof(double)))) {
     release(p);
     delete p;
@@ -614,9 +619,14 @@ int pf_plan_create(const pf_desc* desc, pf_plan** out) {
       return rc;
     }
   }
+  p->nact = p->B;
   std::vector<unsigned long long> seeds(B);
   for (long long i = 0; i < B; ++i) seeds[i] = (unsigned long long)i;
   hipError_t e = hipMemcpy(p->seeds, seeds.data(), B * sizeof(unsigned long long), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    std::vector<long long> z(B, 0);
+    e = hipMemcpy(p->msteps, z.data(), B * sizeof(long long), hipMemcpyHostToDevice);
+  }
   if (e != hipSuccess) {
     release(p);
     delete p;
@@ -655,6 +665,31 @@ int pf_set_seeds(pf_plan* p, const uint64_t* seeds) {
 int pf_set_step_counter(pf_plan* p, int64_t step) {
   if (!p) return fail(PF_EINVAL, "null plan");
   p->step = step;
+  std::vector<long long> v((size_t)p->B, (long long)step);
+  PF_HIP(hipMemcpy(p->msteps, v.data(), v.size() * sizeof(long long), hipMemcpyHostToDevice));
+  return PF_OK;
+}
+
+int pf_set_member(pf_plan* p, int32_t member, uint64_t seed, int64_t step) {
+  if (!p) return fail(PF_EINVAL, "null plan");
+  if (member < 0 || member >= p->B) return fail(PF_EINVAL, "member out of range");
+  const unsigned long long sd = seed;
+  const long long st = step;
+  PF_HIP(hipMemcpy(p->seeds + member, &sd, sizeof(sd), hipMemcpyHostToDevice));
+  PF_HIP(hipMemcpy(p->msteps + member, &st, sizeof(st), hipMemcpyHostToDevice));
+  return PF_OK;
+}
+
+int pf_set_active(pf_plan* p, int32_t n) {
+  if (!p) return fail(PF_EINVAL, "null plan");
+  if (n < 0 || n > p->B) return fail(PF_EINVAL, "active member count out of range");
+  p->nact = n;
+  return PF_OK;
+}
+
+int pf_get_member_steps(pf_plan* p, int64_t* out) {
+  if (!p || !out) return fail(PF_EINVAL, "null argument");
+  PF_HIP(hipMemcpy(out, p->msteps, (size_t)p->B * sizeof(long long), hipMemcpyDeviceToHost));
   return PF_OK;
 }
 
@@ -687,7 +722,7 @@ int pf_set_correlation(pf_plan* p, const void* C, void* stream) {
 
 int pf_forward(pf_plan* p, const void* u, void* uhat, int32_t ncomp, void* stream) {
   if (!p || !u || !uhat || ncomp < 1) return fail(PF_EINVAL, "invalid argument");
-  const int nf = p->B * ncomp;
+  const int nf = p->nact * ncomp;
   return p->fp64 ? forward_fields<double>(p, u, uhat, nf, (hipStream_t)stream)
                  : forward_fields<float>(p, u, uhat, nf, (hipStream_t)stream);
 }
@@ -696,7 +731,7 @@ int pf_backward(pf_plan* p, const void* uhat, void* u, int32_t ncomp, void* stre
   if (!p || !u || !uhat || ncomp < 1) return fail(PF_EINVAL, "invalid argument");
   // the workspace holds batch * dim fields: transform in chunks
   const long long cap = (long long)(p->w1_bytes / ((size_t)p->nk * p->csz));
-  const long long total = (long long)p->B * ncomp;
+  const long long total = (long long)p->nact * ncomp;
   for (long long f0 = 0; f0 < total; f0 += cap) {
     const int nf = (int)std::min(cap, total - f0);
     const char* src = (const char*)uhat + (size_t)f0 * p->nk * p->csz;
@@ -711,16 +746,16 @@ int pf_backward(pf_plan* p, const void* uhat, void* u, int32_t ncomp, void* stre
 int pf_project(pf_plan* p, void* uhat, void* stream) {
   if (!p || !uhat) return fail(PF_EINVAL, "invalid argument");
   const StageArgs g = geometry(p);
-  const int rc = p->fp64 ? launch_project<double>(p->dim, g, uhat, uhat, 0, 0, p->B, (hipStream_t)stream)
-                         : launch_project<float>(p->dim, g, uhat, uhat, 0, 0, p->B, (hipStream_t)stream);
+  const int rc = p->fp64 ? launch_project<double>(p->dim, g, uhat, uhat, 0, 0, p->nact, (hipStream_t)stream)
+                         : launch_project<float>(p->dim, g, uhat, uhat, 0, 0, p->nact, (hipStream_t)stream);
   return rc ? launch_fail(rc, "project") : PF_OK;
 }
 
 int pf_mask_zero_mode(pf_plan* p, void* uhat, void* stream) {
   if (!p || !uhat) return fail(PF_EINVAL, "invalid argument");
   const StageArgs g = geometry(p);
-  const int rc = p->fp64 ? launch_project<double>(p->dim, g, uhat, uhat, 0, 2, p->B, (hipStream_t)stream)
-                         : launch_project<float>(p->dim, g, uhat, uhat, 0, 2, p->B, (hipStream_t)stream);
+  const int rc = p->fp64 ? launch_project<double>(p->dim, g, uhat, uhat, 0, 2, p->nact, (hipStream_t)stream)
+                         : launch_project<float>(p->dim, g, uhat, uhat, 0, 2, p->nact, (hipStream_t)stream);
   return rc ? launch_fail(rc, "mask_zero_mode") : PF_OK;
 }
 
@@ -731,13 +766,14 @@ int pf_rhs_nonlinear(pf_plan* p, const void* uhat, void* rhs, void* stream) {
   int rc = p->fp64 ? nonlinear_unprojected<double>(p, uhat, s) : nonlinear_unprojected<float>(p, uhat, s);
   if (rc) return rc;
   const StageArgs g = geometry(p);
-  rc = p->fp64 ? launch_project<double>(p->dim, g, p->w1, rhs, p->d.mask_nyquist, 0, p->B, s)
-               : launch_project<float>(p->dim, g, p->w1, rhs, p->d.mask_nyquist, 0, p->B, s);
+  rc = p->fp64 ? launch_project<double>(p->dim, g, p->w1, rhs, p->d.mask_nyquist, 0, p->nact, s)
+               : launch_project<float>(p->dim, g, p->w1, rhs, p->d.mask_nyquist, 0, p->nact, s);
   return rc ? launch_fail(rc, "project") : PF_OK;
 }
 
 int pf_step(pf_plan* p, void* uhat, int64_t n_steps, const void* noise, void* stream) {
   if (!p || !uhat || n_steps < 0) return fail(PF_EINVAL, "invalid argument");
+  if (p->nact == 0 || n_steps == 0) return PF_OK;
   if (p->d.noise_type == PF_NOISE_CORRELATED && p->d.noise_mag != 0.0 && uses_noise(p->d.integrator) && !p->corr)
     return fail(PF_EINVAL, "correlated noise needs pf_set_correlation first");
   return p->fp64 ? step_impl<double>(p, uhat, n_steps, noise, (hipStream_t)stream)
@@ -775,8 +811,8 @@ int pf_stage_update(pf_plan* p, int32_t stage, void* unew, const void* u0, const
 
 int pf_draw_noise(pf_plan* p, void* out, void* stream) {
   if (!p || !out) return fail(PF_EINVAL, "invalid argument");
-  const int rc = p->fp64 ? launch_draw<double>(out, p->seeds, p->step, p->nk, p->nc, p->B, (hipStream_t)stream)
-                         : launch_draw<float>(out, p->seeds, p->step, p->nk, p->nc, p->B, (hipStream_t)stream);
+  const int rc = p->fp64 ? launch_draw<double>(out, p->seeds, p->msteps, p->nk, p->nc, p->nact, (hipStream_t)stream)
+                         : launch_draw<float>(out, p->seeds, p->msteps, p->nk, p->nc, p->nact, (hipStream_t)stream);
   return rc ? launch_fail(rc, "draw_noise") : PF_OK;
 }
 
@@ -819,8 +855,8 @@ int pf_diagnostics(pf_plan* p, const void* uhat, double* out, void* stream) {
   a.uhat = uhat; a.out = out; a.nk = p->nk;
   a.N0 = p->N[0]; a.N1 = p->N[1]; a.N2 = p->dim == 3 ? p->N[2] : p->N[1]; a.nh = p->nh;
   a.ks0 = p->ks[0]; a.ks1 = p->ks[1]; a.ks2 = p->ks[2];
-  const int rc = p->fp64 ? launch_diag<double>(p->dim, a, p->B, (hipStream_t)stream)
-                         : launch_diag<float>(p->dim, a, p->B, (hipStream_t)stream);
+  const int rc = p->fp64 ? launch_diag<double>(p->dim, a, p->nact, (hipStream_t)stream)
+                         : launch_diag<float>(p->dim, a, p->nact, (hipStream_t)stream);
   return rc ? launch_fail(rc, "diagnostics") : PF_OK;
 }
 
@@ -830,15 +866,15 @@ int pf_spectrum(pf_plan* p, const void* uhat, double* out, int32_t nshell, void*
   a.uhat = uhat; a.out = out; a.nk = p->nk;
   a.N0 = p->N[0]; a.N1 = p->N[1]; a.N2 = p->dim == 3 ? p->N[2] : p->N[1]; a.nh = p->nh;
   a.ks0 = p->ks[0]; a.ks1 = p->ks[1]; a.ks2 = p->ks[2];
-  const int rc = p->fp64 ? launch_spectrum<double>(p->dim, a, p->B, nshell, (hipStream_t)stream)
-                         : launch_spectrum<float>(p->dim, a, p->B, nshell, (hipStream_t)stream);
+  const int rc = p->fp64 ? launch_spectrum<double>(p->dim, a, p->nact, nshell, (hipStream_t)stream)
+                         : launch_spectrum<float>(p->dim, a, p->nact, nshell, (hipStream_t)stream);
   return rc ? launch_fail(rc, "spectrum") : PF_OK;
 }
 
 int pf_accumulate_sk(pf_plan* p, const void* uhat, double* acc, void* stream) {
   if (!p || !uhat || !acc) return fail(PF_EINVAL, "invalid argument");
-  const int rc = p->fp64 ? launch_sk<double>(p->dim, uhat, acc, p->nk, p->B, (hipStream_t)stream)
-                         : launch_sk<float>(p->dim, uhat, acc, p->nk, p->B, (hipStream_t)stream);
+  const int rc = p->fp64 ? launch_sk<double>(p->dim, uhat, acc, p->nk, p->nact, (hipStream_t)stream)
+                         : launch_sk<float>(p->dim, uhat, acc, p->nk, p->nact, (hipStream_t)stream);
   return rc ? launch_fail(rc, "accumulate_sk") : PF_OK;
 }
 

@@ -44,7 +44,7 @@ template <typename T, int MODE> constexpr int row_pairs(int L) {
   int p = (sizeof(T) == 4 ? 4096 : 2048) / (row_slots<MODE>() * L);
   return p < 1 ? 1 : (p > 8 ? 8 : p);
 }
-__host__ __device__ constexpr int row_pitch(int L) { return L + L / 16 + 1; }
+__host__ __device__ constexpr int row_pitch(int L) { return L + L / 16; }
 
 // Hermitian extension of a stored half spectrum to index k of a length-L
 // complex transform.  DC keeps its real part only; the stored Nyquist entry
@@ -325,15 +325,15 @@ __device__ __forceinline__ void rng_draw(unsigned long long seed, long long step
 
 // raw draws (B, 2, nc, nk) of one step: the device counterpart of random_fields
 template <typename T>
-__global__ void __launch_bounds__(256) draw_kernel(void* out, const unsigned long long* seeds, long long step,
-                                                   long long nk, int nc) {
+__global__ void __launch_bounds__(256) draw_kernel(void* out, const unsigned long long* seeds,
+                                                   const long long* msteps, long long nk, int nc) {
   const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
   const int b = blockIdx.y;
   if (idx >= nk) return;
   cplx<T>* o = (cplx<T>*)out + (long long)b * 2 * nc * nk + idx;
   for (int c = 0; c < nc; ++c) {
     cplx<T> w1, w2;
-    rng_draw<T>(seeds[b], step, idx, c, w1, w2);
+    rng_draw<T>(seeds[b], msteps[b], idx, c, w1, w2);
     o[(long long)c * nk] = w1;
     o[(long long)(nc + c) * nk] = w2;
   }
@@ -363,6 +363,7 @@ struct StageArgs {
   int mask_nyq;
   double fk2min, fk2max;
   long long step;
+  const long long* msteps;     // per-member step counters keying the RNG (null: step)
   // mode 0: fused stage; 1: write the four right-hand-side terms (compute_rhs);
   // 2: stage update from given terms (an integrator's stepping)
   int mode;
@@ -510,7 +511,7 @@ __device__ __forceinline__ void stage_point(const StageArgs& a, long long idx, i
       if (c < nc) {
         cplx<T> w1, w2;
         if (raw) { w1 = raw[(long long)c * a.nk]; w2 = raw[(long long)(nc + c) * a.nk]; }
-        else rng_draw<T>(seed, a.step, idx, c, w1, w2);
+        else rng_draw<T>(seed, a.msteps ? a.msteps[b] : a.step, idx, c, w1, w2);
         w[c] = add<T>(scl<T>(w1, fac), scl<T>(scl<T>(w2, fac), na));
       }
     }

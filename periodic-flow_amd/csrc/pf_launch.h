@@ -29,7 +29,8 @@ int launch_project(int dim, const StageArgs& geo, const void* in, void* out, int
                    int n_members, hipStream_t s);
 int launch_fill(double* p, long long n, double v, hipStream_t s);
 template <typename T>
-int launch_draw(void* out, const unsigned long long* seeds, long long step, long long nk, int nc, int nmem,
+int launch_draw(void* out, const unsigned long long* seeds, const long long* msteps, long long nk, int nc, int nmem,
                 hipStream_t s);
+int launch_member_step(long long* msteps, int n, hipStream_t s);
 
 }  // namespace pf

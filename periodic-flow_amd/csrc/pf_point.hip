@@ -3,6 +3,12 @@
 
 namespace pf {
 
+// one step done: advance the RNG step counter of every active member
+__global__ void member_step_kernel(long long* msteps, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) msteps[i] += 1;
+}
+
 __global__ void fill_kernel(double* p, long long n, double v) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -95,10 +101,16 @@ template <typename T> int launch_sk(int dim, const void* u, double* acc, long lo
 }
 
 template <typename T>
-int launch_draw(void* out, const unsigned long long* seeds, long long step, long long nk, int nc, int nmem,
+int launch_draw(void* out, const unsigned long long* seeds, const long long* msteps, long long nk, int nc, int nmem,
                 hipStream_t s) {
   dim3 grid((unsigned)((nk + 255) / 256), (unsigned)nmem);
-  hipLaunchKernelGGL((draw_kernel<T>), grid, dim3(256), 0, s, out, seeds, step, nk, nc);
+  hipLaunchKernelGGL((draw_kernel<T>), grid, dim3(256), 0, s, out, seeds, msteps, nk, nc);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+int launch_member_step(long long* msteps, int n, hipStream_t s) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(member_step_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, msteps, n);
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
@@ -109,7 +121,7 @@ int launch_draw(void* out, const unsigned long long* seeds, long long step, long
   template int launch_spectrum<T>(int, const DiagArgs&, int, int, hipStream_t);                       \
   template int launch_band<T>(int, const BandArgs&, int, hipStream_t);                                \
   template int launch_sk<T>(int, const void*, double*, long long, int, hipStream_t);                  \
-  template int launch_draw<T>(void*, const unsigned long long*, long long, long long, int, int, hipStream_t);
+  template int launch_draw<T>(void*, const unsigned long long*, const long long*, long long, int, int, hipStream_t);
 PF_INST(float)
 PF_INST(double)
 #undef PF_INST

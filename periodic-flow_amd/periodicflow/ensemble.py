@@ -1,0 +1,167 @@
+"""Batched ensembles: many independent members, of uneven grid sizes, on one GPU.
+
+Members with the same shape and parameters share one plan (one launch per
+pass for all of them); distinct shapes form separate groups that are
+launched back to back on one stream, or replayed from one captured HIP graph
+per step.  Every member keeps its own seed and RNG step counter, so its
+result does not depend on the other members.  The reference runs one solve
+per MPI job; this layer has no counterpart there.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from .plan import Plan
+
+
+@dataclass(frozen=True)
+class MemberSpec:
+    """Parameters of one member (the Solver keywords that shape a plan)."""
+
+    n: tuple
+    dt: float
+    integrator: str = "rk3"
+    viscosity: float = 1e-3
+    nonlinear: bool = True
+    dealias: str = "3/2"
+    noise_type: str = "thermal"
+    noise_mag: float = 0.0
+    forcing: str | None = None
+    forcing_power: float = 0.1
+    forcing_band: tuple = (1.0, 2.5)
+    length: tuple | None = None
+
+    def key(self, precision):
+        return (tuple(self.n), self.dt, self.integrator, self.viscosity, self.nonlinear, self.dealias,
+                self.noise_type, self.noise_mag, self.forcing, self.forcing_power, tuple(self.forcing_band),
+                None if self.length is None else tuple(self.length), precision)
+
+
+@dataclass
+class Group:
+    spec: MemberSpec
+    plan: Plan
+    state: torch.Tensor
+    members: list = field(default_factory=list)  # global member ids, slot order
+
+
+class Ensemble:
+    """A set of members advanced together.
+
+    ``members``: list of (MemberSpec, seed).  States are device tensors, one
+    (batch, d, *Nf) tensor per group; ``state(i)`` is member i's view.
+    """
+
+    def __init__(self, members, precision="fp32", correlation=None):
+        self.precision = precision
+        self.groups: dict = {}
+        self.where = []  # member id -> (group key, slot)
+        order = {}
+        for i, (spec, seed) in enumerate(members):
+            k = spec.key(precision)
+            order.setdefault(k, []).append((i, spec, seed))
+        self.where = [None] * len(members)
+        for k, lst in order.items():
+            spec = lst[0][1]
+            plan = Plan(spec.n, spec.dt, length=spec.length, precision=precision, dealias=spec.dealias,
+                        integrator=spec.integrator, nonlinear=spec.nonlinear, viscosity=spec.viscosity,
+                        noise_type=spec.noise_type, noise_mag=spec.noise_mag, forcing=spec.forcing,
+                        forcing_power=spec.forcing_power, forcing_band=spec.forcing_band, batch=len(lst),
+                        seeds=[s for _, _, s in lst])
+            if spec.noise_type == "correlated" and spec.noise_mag != 0:
+                if correlation is None:
+                    raise ValueError("correlated noise needs a correlation function: correlation(spec) -> array")
+                C = torch.as_tensor(np.asarray(correlation(spec)), device="cuda", dtype=plan.complex_dtype)
+                plan.set_correlation(C.contiguous())
+            g = Group(spec, plan, plan.new_state(), [i for i, _, _ in lst])
+            self.groups[k] = g
+            for slot, (i, _, _) in enumerate(lst):
+                self.where[i] = (k, slot)
+        self._graph = None
+
+    def __len__(self):
+        return len(self.where)
+
+    def state(self, i):
+        k, slot = self.where[i]
+        return self.groups[k].state[slot]
+
+    def plan_of(self, i):
+        return self.groups[self.where[i][0]].plan
+
+    def set_fourier(self, i, u_hat):
+        self.state(i).copy_(torch.as_tensor(u_hat, device="cuda", dtype=self.plan_of(i).complex_dtype))
+
+    def points_per_step(self):
+        return sum(int(np.prod(g.spec.n)) * g.plan.batch for g in self.groups.values())
+
+    def step(self, n_steps=1):
+        for g in self.groups.values():
+            g.plan.step(g.state, n_steps)
+
+    # ---------------------------------------------------------------- graphs
+    def capture(self):
+        """Capture one step of every group into a HIP graph (replayed by run)."""
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            for grp in self.groups.values():
+                grp.plan.step(grp.state, 1)
+        self._graph = g
+        return g
+
+    def run(self, n_steps, graph=False):
+        if graph:
+            if self._graph is None:
+                self.capture()
+                # capture records the step without running it
+            for _ in range(n_steps):
+                self._graph.replay()
+        else:
+            self.step(n_steps)
+
+    def diagnostics(self):
+        """[(energy, enstrophy)] per member, FP64."""
+        out = [None] * len(self.where)
+        for g in self.groups.values():
+            d = g.plan.diagnostics(g.state).cpu().numpy()
+            for slot, i in enumerate(g.members):
+                out[i] = (float(d[slot, 0]), float(d[slot, 1]))
+        return out
+
+
+def random_solenoidal(plan, seed, energy=0.5, kp=4.0):
+    """Seeded solenoidal field with E(k) ~ k^4 exp(-2 k^2 / kp^2), total energy ``energy``.
+
+    Built on the host in FP64 (a set-up step, not the time step) and
+    returned as one member's (d, *Nf) spectrum.
+    """
+    rs = np.random.RandomState(seed)
+    d = plan.dim
+    n = plan.n
+    ks = []
+    for i, ni in enumerate(n):
+        k = np.fft.fftfreq(ni, 1.0 / ni) if i < d - 1 else np.fft.rfftfreq(ni, 1.0 / ni)
+        shp = [1] * d
+        shp[i] = k.size
+        ks.append(k.reshape(shp))
+    kv = np.array(np.broadcast_arrays(*ks))
+    k2 = np.sum(kv**2, axis=0)
+    kk = np.sqrt(k2)
+    u = rs.standard_normal((d,) + plan.shape_fourier) + 1j * rs.standard_normal((d,) + plan.shape_fourier)
+    amp = np.where(kk > 0, kk**2 * np.exp(-kk**2 / kp**2), 0.0) / np.maximum(kk, 1e-30) ** ((d - 1) / 2)
+    u *= amp
+    # project
+    p = np.sum(kv * u, axis=0) / np.where(k2 == 0, 1, k2)
+    u -= kv * p
+    w = np.full(plan.shape_fourier, 2.0)
+    w[..., 0] = 1.0
+    if n[-1] % 2 == 0:
+        w[..., -1] = 1.0
+    e = 0.5 * np.sum(w * np.sum(np.abs(u) ** 2, axis=0))
+    u *= math.sqrt(energy / e) if e > 0 else 0.0
+    return u

@@ -85,6 +85,7 @@ class Plan:
         self.nonlinear = bool(nonlinear)
         self.noise_mag = float(noise_mag)
         self.noise_type = noise_type
+        self._active = self.batch
         if seeds is not None:
             self.set_seeds(seeds)
 
@@ -115,7 +116,8 @@ class Plan:
         return (self.batch, self.dim if ncomp is None else ncomp) + self.shape_physical
 
     def noise_shape(self):
-        return (self.batch, 2, self.noise_components) + self.shape_fourier
+        """Raw draws of one step for the active members."""
+        return (self._active, 2, self.noise_components) + self.shape_fourier
 
     def new_state(self, device="cuda"):
         return torch.zeros(self.state_shape(), dtype=self.complex_dtype, device=device)
@@ -132,6 +134,25 @@ class Plan:
     @step_counter.setter
     def step_counter(self, v):
         call("pf_set_step_counter", self._h, int(v))
+
+    @property
+    def active(self):
+        return self._active
+
+    @active.setter
+    def active(self, n):
+        """Advance only members [0, n) (0 <= n <= batch)."""
+        call("pf_set_active", self._h, int(n))
+        self._active = int(n)
+
+    def set_member(self, member, seed, step=0):
+        """Seed and RNG step counter of one slot."""
+        call("pf_set_member", self._h, int(member), int(seed) & 0xFFFFFFFFFFFFFFFF, int(step))
+
+    def member_steps(self):
+        arr = (C.c_int64 * self.batch)()
+        call("pf_get_member_steps", self._h, arr)
+        return [int(v) for v in arr]
 
     def set_linear_operator(self, L):
         if L is None:
