@@ -70,6 +70,102 @@ def cpu_baseline(steps):
     }
 
 
+def _rate(torch, fn, points):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fn()
+    torch.cuda.synchronize()
+    return points / (time.perf_counter() - t0) / 1e6
+
+
+def batched_vs_plain(torch, members=256, steps=20, plain_members=32):
+    """Batched FP32 ensemble stepping vs the plain path (the same kernels at
+    batch 1, one solve after another), on the 2D thermal-noise Stokes
+    ensemble (512^2, implicit PC, dt = 0.05, nu = 1, kT = 1e-3, seeds
+    1000 + r) and on a small 3D forced-turbulence ensemble (64^3, RK3)."""
+    import math
+
+    from periodicflow.ensemble import random_solenoidal
+    from periodicflow.plan import Plan
+
+    out = {}
+    n, nu, dt, kT = 512, 1.0, 0.05, 1e-3
+    nm = math.sqrt(2 * nu * kT / (2 * math.pi / n) ** 2)
+    kw = dict(precision="fp32", integrator="implicit_pc", nonlinear=False, viscosity=nu, noise_mag=nm)
+    pb = Plan((n, n), dt, batch=members, seeds=[1000 + r for r in range(members)], **kw)
+    ub = pb.new_state()
+    pb.step(ub, 2)
+    ub.zero_()
+    pb.step_counter = 0
+    rb = _rate(torch, lambda: pb.step(ub, steps), n * n * members * steps)
+    pp = Plan((n, n), dt, batch=1, **kw)
+    up = pp.new_state()
+    pp.step(up, 2)
+
+    def plain():
+        for m in range(plain_members):
+            pp.set_member(0, 1000 + m, 0)
+            up.zero_()
+            pp.step(up, steps)
+
+    rp = _rate(torch, plain, n * n * plain_members * steps)
+    p64 = Plan((n, n), dt, batch=1, seeds=[1000], **dict(kw, precision="fp64"))
+    u64 = p64.new_state()
+    p64.step(u64, steps)
+    e = float(torch.linalg.vector_norm(ub[0].to(torch.complex128) - u64[0]) / torch.linalg.vector_norm(u64[0]))
+    out["stokes2d_512_thermal_implicit_pc"] = {
+        "members": members, "steps": steps, "batched": round(rb, 1), "plain": round(rp, 1),
+        "plain_members_timed": plain_members, "speedup": round(rb / rp, 2), "fp32_rel_l2_vs_fp64": e}
+    del pb, ub, pp, up, p64, u64
+
+    m3, n3, st3 = 32, 64, 10
+    kw3 = dict(precision="fp32", integrator="rk3", viscosity=0.005, forcing="constant_power", forcing_power=0.1)
+    pb = Plan((n3,) * 3, 5e-3, batch=m3, seeds=list(range(m3)), **kw3)
+    ub = pb.new_state()
+    inits = [random_solenoidal(pb, 2000 + m, energy=0.5) for m in range(m3)]
+    for m in range(m3):
+        ub[m].copy_(torch.as_tensor(inits[m], device="cuda", dtype=pb.complex_dtype))
+    u_start = ub.clone()
+    pb.step(ub, 1)
+    ub.copy_(u_start)
+    rb = _rate(torch, lambda: pb.step(ub, st3), n3**3 * m3 * st3)
+    pp = Plan((n3,) * 3, 5e-3, batch=1, **kw3)
+    up = pp.new_state()
+
+    def plain3():
+        for m in range(8):
+            up.copy_(u_start[m:m + 1])
+            pp.step(up, st3)
+
+    pp.step(up, 1)
+    rp = _rate(torch, plain3, n3**3 * 8 * st3)
+    out["forced3d_64_rk3"] = {"members": m3, "steps": st3, "batched": round(rb, 1), "plain": round(rp, 1),
+                              "plain_members_timed": 8, "speedup": round(rb / rp, 2)}
+    return out
+
+
+def suite(torch, path):
+    """The long configurations, written to ``path`` (one JSON object)."""
+    from periodicflow import workloads as W
+    from periodicflow.serving import Engine, GpuBackend, serve_one_at_a_time
+
+    res = {}
+    r = W.tgv64_rk3(200)
+    res["tgv64_rk3_fp64"] = {"E0": float(r["energy"][0]), "E200": float(r["energy"][-1]),
+                             "eps0": float(r["dissipation"][0]), "eps200": float(r["dissipation"][-1])}
+    res["stokes512_thermal"] = W.stokes512()
+    f = W.forced3d(steps=100)
+    f.pop("ensemble")
+    res["forced3d_16x128_16x96"] = f
+    reqs = W.serving_requests(count=60, max_steps=400)
+    res["serving_batched"] = Engine(GpuBackend("fp32"), hbm_budget=96 << 30, keep_results=False).run(reqs)
+    reqs = W.serving_requests(count=60, max_steps=400)
+    res["serving_one_at_a_time"] = serve_one_at_a_time(GpuBackend("fp32"), reqs, hbm_budget=96 << 30)[0]
+    with open(path, "w") as fh:
+        json.dump(res, fh, indent=1, default=float)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +174,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-check", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--suite", default=None, help="also run the long configurations, results to this JSON file")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 1 GiB device-to-device copy (the counter calibration kernel)")
     args = ap.parse_args()
@@ -189,6 +287,11 @@ def main():
         "fp32_rel_l2_vs_fp64": err,
         "energy_end": energy32,
     }
+    if not args.no_batched:
+        out["batched_vs_plain"] = batched_vs_plain(torch)
+    if args.suite:
+        suite(torch, args.suite)
+        out["suite"] = args.suite
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
     print(json.dumps(out), flush=True)

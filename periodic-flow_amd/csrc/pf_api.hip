@@ -483,14 +483,20 @@ template <typename T> int step_impl(pf_plan* p, void* uhat, long long n_steps, c
   const bool noise_on = p->d.noise_mag != 0.0 && uses_noise(p->d.integrator);
   const long long noise_block = (long long)p->nact * 2 * p->nc * p->nk;  // complex per step
   const bool fused = p->fuse_stage && col_stage_width<T>(p->M[0]) > 0;
+  // Stokes flow with implicit PC and no forcing: the predictor's result is
+  // never read (the corrector uses u0, L u0 and the noise only), so the step
+  // is the corrector alone.
+  const int first = (!p->d.nonlinear && p->d.forcing == PF_FORCING_NONE && p->d.integrator == PF_IMPLICIT_PC &&
+                     p->fuse_stage) ? 1 : 0;
   for (long long st = 0; st < n_steps; ++st) {
     const void* nz = noise ? (const char*)noise + (size_t)(st * noise_block) * p->csz : nullptr;
-    for (int stage = 0; stage < ns; ++stage) {
+    for (int stage = first; stage < ns; ++stage) {
       if (p->d.forcing == PF_FORCING_CONSTANT_POWER) {
         int rc = band_energy<T>(p, uhat, s);
         if (rc) return rc;
       }
       StageArgs a = stage_args(p, uhat, stage, nz, noise_on);
+      if (stage == first && first > 0) a.save_u0 = 2;  // u_hat is still u0; nothing to save
       if (p->d.nonlinear && fused) {
         int rc = nonlinear_front<T>(p, uhat, s);
         if (rc) return rc;

@@ -433,7 +433,7 @@ __device__ __forceinline__ void stage_point(const StageArgs& a, long long idx, i
     nz[i] = mk<T>(0, 0);
     fr[i] = mk<T>(0, 0);
   }
-  if (a.save_u0 && a.mode == 0) {
+  if (a.save_u0 == 1 && a.mode == 0) {
 #pragma unroll
     for (int i = 0; i < D; ++i) u0p[i * a.nk] = u[i];
   }
@@ -506,13 +506,36 @@ __device__ __forceinline__ void stage_point(const StageArgs& a, long long idx, i
     const int nc = a.noise_type == 0 ? D * D : D;
     const cplx<T>* raw = a.noise ? (const cplx<T>*)a.noise + (long long)b * 2 * nc * a.nk + idx : nullptr;
     cplx<T> w[D * D];
+    const long long stp = a.msteps ? a.msteps[b] : a.step;
+    if (!raw && a.noise_type == 0) {
+      // On-device stream, thermal noise: draw the independent entries of the
+      // symmetric tensor directly.  W_ij + W_ji over sqrt 2 of two independent
+      // draws has the law of one draw, so (W + W^T) sqrt(1/2) is sampled with
+      // D(D+1)/2 draws per mode instead of D^2, with the same distribution.
+      int c = 0;
 #pragma unroll
-    for (int c = 0; c < D * D; ++c) {
-      if (c < nc) {
-        cplx<T> w1, w2;
-        if (raw) { w1 = raw[(long long)c * a.nk]; w2 = raw[(long long)(nc + c) * a.nk]; }
-        else rng_draw<T>(seed, a.msteps ? a.msteps[b] : a.step, idx, c, w1, w2);
-        w[c] = add<T>(scl<T>(w1, fac), scl<T>(scl<T>(w2, fac), na));
+      for (int i = 0; i < D; ++i) {
+#pragma unroll
+        for (int j = i; j < D; ++j) {
+          cplx<T> w1, w2;
+          rng_draw<T>(seed, stp, idx, c++, w1, w2);
+          const cplx<T> g = add<T>(scl<T>(w1, fac), scl<T>(scl<T>(w2, fac), na));
+          // stored so that the symmetrisation below reproduces g off the
+          // diagonal and sqrt(2) g on it
+          const T h = (T)0.70710678118654752440;
+          w[i * D + j] = scl<T>(g, i == j ? (T)1 : h);
+          w[j * D + i] = w[i * D + j];
+        }
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < D * D; ++c) {
+        if (c < nc) {
+          cplx<T> w1, w2;
+          if (raw) { w1 = raw[(long long)c * a.nk]; w2 = raw[(long long)(nc + c) * a.nk]; }
+          else rng_draw<T>(seed, stp, idx, c, w1, w2);
+          w[c] = add<T>(scl<T>(w1, fac), scl<T>(scl<T>(w2, fac), na));
+        }
       }
     }
     if (a.noise_type == 0) {

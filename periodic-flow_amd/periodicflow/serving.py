@@ -142,7 +142,7 @@ class GpuBackend:
 
 class Engine:
     def __init__(self, backend, hbm_budget=64 << 30, chunk=32, max_slots=64, keep_results=True,
-                 clock=time.perf_counter):
+                 clock=time.perf_counter, policy="srpt"):
         self.be = backend
         self.budget = int(hbm_budget)
         self.chunk = int(chunk)
@@ -154,6 +154,9 @@ class Engine:
         self.finished: list = []
         self.t0 = None
         self.in_use = 0
+        if policy not in ("srpt", "all"):
+            raise ValueError("policy must be 'srpt' or 'all'")
+        self.policy = policy
 
     # ------------------------------------------------------------ admission
     def _capacity(self, spec):
@@ -241,9 +244,20 @@ class Engine:
         self.finished.append(r)
         pool.last_used = r.t_done
 
+    def _order(self):
+        """Pools to advance this round.  'all': every pool with members.
+        'srpt': the pool whose shortest member has the least remaining work
+        (point-steps); members of a pool always advance together."""
+        act = [p for p in self.pools.values() if p.slots]
+        if self.policy == "all" or len(act) <= 1:
+            return act
+        def rem(p):
+            return min((r.steps - r.steps_done) for r in p.slots) * int(np.prod(p.spec.n))
+        return [min(act, key=rem)]
+
     def _advance(self):
         busy = False
-        for pool in list(self.pools.values()):
+        for pool in self._order():
             n = len(pool.slots)
             if n == 0:
                 continue
