@@ -132,23 +132,32 @@ int make_twiddles(pf_plan* p, int L) {
   void* dptr = nullptr;
   int rc = dev_alloc(p, &dptr, (size_t)L * p->csz);
   if (rc) return rc;
+  // per-pass blocks: entry tw_offset(L, p) + (r - 1) Ns + k = exp(-2 pi i r k / (Ns R))
   const long double twopi = 6.283185307179586476925286766559005768L;
+  const RadixPlan pl = make_plan(L);
+  std::vector<long double> re, im;
+  int ns = 1;
+  for (int q = 0; q < pl.n; ++q) {
+    const int R = pl.r[q];
+    if (q > 0) {
+      for (int r = 1; r < R; ++r)
+        for (int k = 0; k < ns; ++k) {
+          const long double ang = twopi * (long double)(r * k) / (long double)(ns * R);
+          re.push_back(cosl(ang));
+          im.push_back(-sinl(ang));
+        }
+    }
+    ns *= R;
+  }
+  if ((int)re.size() != tw_size(L)) return fail(PF_EINVAL, "twiddle table size mismatch");
   if (p->fp64) {
-    std::vector<double> h(2 * L);
-    for (int m = 0; m < L; ++m) {
-      const long double ang = twopi * (long double)m / (long double)L;
-      h[2 * m] = (double)cosl(ang);
-      h[2 * m + 1] = (double)-sinl(ang);
-    }
-    PF_HIP(hipMemcpy(dptr, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    std::vector<double> h(2 * re.size() + 2, 0.0);
+    for (size_t m = 0; m < re.size(); ++m) { h[2 * m] = (double)re[m]; h[2 * m + 1] = (double)im[m]; }
+    PF_HIP(hipMemcpy(dptr, h.data(), 2 * re.size() * sizeof(double), hipMemcpyHostToDevice));
   } else {
-    std::vector<float> h(2 * L);
-    for (int m = 0; m < L; ++m) {
-      const long double ang = twopi * (long double)m / (long double)L;
-      h[2 * m] = (float)cosl(ang);
-      h[2 * m + 1] = (float)-sinl(ang);
-    }
-    PF_HIP(hipMemcpy(dptr, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+    std::vector<float> h(2 * re.size() + 2, 0.f);
+    for (size_t m = 0; m < re.size(); ++m) { h[2 * m] = (float)re[m]; h[2 * m + 1] = (float)im[m]; }
+    PF_HIP(hipMemcpy(dptr, h.data(), 2 * re.size() * sizeof(float), hipMemcpyHostToDevice));
   }
   p->tw[L] = dptr;
   return 0;
@@ -215,7 +224,14 @@ template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStrea
       a.n2k = N2 / 3 + 1;
       a.cols = (2 * a.K1 + 1) * a.n2k;
     }
-    { PassTimer t(p, PF_PASS_INV0, s); PF_TRY(col<T>(p, M0, K_INV_CURL3, a, s)); }
+    a.n_fields = 3;
+    {
+      PassTimer t(p, PF_PASS_INV0, s);
+      a.f_base = 0;
+      PF_TRY(col<T>(p, M0, K_INV_U3, a, s));
+      a.f_base = 3;
+      PF_TRY(col<T>(p, M0, K_INV_CURL3, a, s));
+    }
     // P2: axis-1 inverse, A -> Bbuf (w2)
     a = col_args();
     a.in = p->w1; a.out = p->w2;
@@ -267,7 +283,15 @@ template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStrea
     a.ks0 = p->ks[0]; a.ks1 = p->ks[1];
     a.mask23 = pr;
     if (pr) a.cols = N1 / 3 + 1;
-    { PassTimer t(p, PF_PASS_INV0, s); PF_TRY(col<T>(p, M0, K_INV_CURL2, a, s)); }
+    {
+      PassTimer t(p, PF_PASS_INV0, s);
+      a.n_fields = 2;
+      a.f_base = 0;
+      PF_TRY(col<T>(p, M0, K_INV_U2, a, s));
+      a.n_fields = 1;
+      a.f_base = 2;
+      PF_TRY(col<T>(p, M0, K_INV_CURL2, a, s));
+    }
     // P2: fused last axis: A -> C (2 fields, w2)
     RowArgs r{};
     r.in = p->w1; r.out = p->w2;

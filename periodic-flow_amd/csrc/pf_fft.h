@@ -145,7 +145,22 @@ template <int R, int S, typename T> __device__ __forceinline__ void dft(cplx<T>*
   else if constexpr (R == 16) dft16<S, T>(v);
 }
 
-// twiddle table entry m of exp(-2 pi i m / L), conjugated for the inverse
+// Twiddle table, one block per pass p >= 1 (pass 0 has span 1 and none):
+// entry OFF(p) + (r - 1) Ns + k = exp(-2 pi i r k / (Ns R)).  Lanes that
+// take consecutive butterflies read consecutive entries.
+constexpr int tw_offset(int L, int P) {
+  RadixPlan pl = make_plan(L);
+  int off = 0, ns = 1;
+  for (int q = 0; q < P; ++q) {
+    if (q > 0) off += (pl.r[q] - 1) * ns;
+    ns *= pl.r[q];
+  }
+  return off;
+}
+constexpr int tw_size(int L) { return tw_offset(L, make_plan(L).n); }
+constexpr int tw_alloc(int L) { return tw_size(L) > 0 ? tw_size(L) : 1; }
+
+// conjugated for the inverse
 template <int S, typename T> __device__ __forceinline__ cplx<T> twiddle(const cplx<T>* tw, int m) {
   cplx<T> w = tw[m];
   if constexpr (S > 0) w.y = -w.y;
@@ -175,8 +190,9 @@ __device__ __forceinline__ void stockham(cplx<T>* buf, const cplx<T>* tw, const 
 #pragma unroll
         for (int r = 0; r < R; ++r) v[kk][r] = buf[addr(f, j + r * NB)];
         if constexpr (NS > 1) {
+          constexpr int OFF = tw_offset(L, P);
 #pragma unroll
-          for (int r = 1; r < R; ++r) v[kk][r] = mul<T>(v[kk][r], twiddle<S, T>(tw, r * k * (L / (NS * R))));
+          for (int r = 1; r < R; ++r) v[kk][r] = mul<T>(v[kk][r], twiddle<S, T>(tw, OFF + (r - 1) * NS + k));
         }
         dft<R, S, T>(v[kk]);
       }

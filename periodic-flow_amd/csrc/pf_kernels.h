@@ -74,9 +74,9 @@ __global__ void __launch_bounds__(kThreads) row_pass(RowArgs a, const cplx<T>* _
   constexpr int NS = row_slots<MODE>();
   constexpr int NP = row_pairs<T, MODE>(L);
   constexpr int LP = row_pitch(L);
-  __shared__ cplx<T> tws[L];
+  __shared__ cplx<T> tws[tw_alloc(L)];
   __shared__ cplx<T> buf[NP * NS * LP];
-  for (int m = threadIdx.x; m < L; m += kThreads) tws[m] = tw[m];
+  for (int m = threadIdx.x; m < tw_size(L); m += kThreads) tws[m] = tw[m];
 
   const int npairs = a.n_rows / 2;
   const int pair0 = blockIdx.x * NP;
@@ -819,7 +819,16 @@ __global__ void __launch_bounds__(kThreads) spectrum_kernel(DiagArgs a, int nshe
 // modes: rows with |k| > Krow are zero without a load, and the axis-0 pass
 // of 3D enumerates only the (j, l) columns with |k_j| <= K1, l <= K2; the
 // columns it skips are never read downstream.
-enum ColKind { K_INV = 0, K_INV_CURL3 = 1, K_INV_CURL2 = 2, K_FWD = 3, K_FWD_STAGE3 = 4, K_FWD_STAGE2 = 5 };
+enum ColKind {
+  K_INV = 0,
+  K_INV_CURL3 = 1,   // vorticity components f_base + fg, fg = 0..2, of a 3D field
+  K_INV_CURL2 = 2,   // scalar vorticity of a 2D field
+  K_FWD = 3,
+  K_FWD_STAGE3 = 4,
+  K_FWD_STAGE2 = 5,
+  K_INV_U3 = 6,      // velocity components of a 3D field (2/3 mask, compact columns)
+  K_INV_U2 = 7,      // velocity components of a 2D field (2/3 mask)
+};
 
 struct ColArgs {
   const void* in;
@@ -841,6 +850,7 @@ struct ColArgs {
   int prune;                    // visit retained modes only (2/3 rule)
   int Krow;                     // pruned inverse: rows with |k| > Krow are zero
   int K1, n2k;                  // pruned 3D axis-0: |k_j| <= K1, l < n2k
+  int f_base;                   // first output field of the launch
   StageArgs st;                 // fused stage kinds
 };
 
@@ -877,15 +887,19 @@ __device__ __forceinline__ long long xcd_order(long long bid, long long G) {
 template <typename T, int L, int TC, int KIND>
 __global__ void __launch_bounds__(kThreads) col_pass(ColArgs a, const cplx<T>* __restrict__ tw) {
   constexpr bool STAGE = KIND == K_FWD_STAGE3 || KIND == K_FWD_STAGE2;
-  constexpr bool INV = KIND == K_INV || KIND == K_INV_CURL3 || KIND == K_INV_CURL2;
+  constexpr bool INV = KIND == K_INV || KIND == K_INV_CURL3 || KIND == K_INV_CURL2 || KIND == K_INV_U3 ||
+                        KIND == K_INV_U2;
   constexpr bool CURL = KIND == K_INV_CURL3 || KIND == K_INV_CURL2;
+  constexpr bool VEL = KIND == K_INV_U3 || KIND == K_INV_U2;
+  constexpr bool GRID = CURL || VEL;  // loads need the mode numbers of the unpadded grid
+  constexpr int GD = (KIND == K_INV_CURL3 || KIND == K_INV_U3) ? 3 : 2;
   constexpr int NFLD = KIND == K_FWD_STAGE3 ? 3 : (KIND == K_FWD_STAGE2 ? 2 : 1);
   constexpr int W = NFLD * TC;  // LDS columns
   constexpr int NIT = (L * W + kThreads - 1) / kThreads;
   constexpr int CH = NIT < 16 ? NIT : 16;  // loads kept in flight per thread
-  __shared__ cplx<T> tws[L];
+  __shared__ cplx<T> tws[tw_alloc(L)];
   __shared__ cplx<T> buf[L * W];
-  for (int m = threadIdx.x; m < L; m += kThreads) tws[m] = tw[m];
+  for (int m = threadIdx.x; m < tw_size(L); m += kThreads) tws[m] = tw[m];
 
   const int nfg = STAGE ? 1 : a.n_fields;
   long long lg = xcd_order(blockIdx.x, gridDim.x);
@@ -900,7 +914,7 @@ __global__ void __launch_bounds__(kThreads) col_pass(ColArgs a, const cplx<T>* _
 
   // compact column -> column of the matrix
   auto colmap = [&](int cc) -> int {
-    if (KIND == K_INV_CURL3 && a.prune) {
+    if ((KIND == K_INV_CURL3 || KIND == K_INV_U3) && a.prune) {
       const int jj = cc / a.n2k, l = cc % a.n2k;
       const int j = jj <= a.K1 ? jj : jj + a.gN1 - (2 * a.K1 + 1);
       return j * a.gnh + l;
@@ -911,16 +925,13 @@ __global__ void __launch_bounds__(kThreads) col_pass(ColArgs a, const cplx<T>* _
   // ---- load: per chunk, issue every load, then combine and store to LDS
   for (int base = 0; base < NIT; base += CH) {
     cplx<T> ra[CH], rb[CH];
-    T ka[CH], kb[CH], fac[CH];
-    bool ok[CH];
+    T ka[CH], kb[CH];
 #pragma unroll
     for (int q = 0; q < CH; ++q) {
       const int idx = threadIdx.x + (base + q) * kThreads;
       ra[q] = mk<T>(0, 0);
       rb[q] = mk<T>(0, 0);
       ka[q] = kb[q] = (T)0;
-      fac[q] = (T)1;
-      ok[q] = false;
       if (base + q >= NIT || idx >= L * W) continue;
       const int e = idx / W, cf = idx % W;
       const int f = STAGE ? cf / TC : fg;
@@ -928,42 +939,32 @@ __global__ void __launch_bounds__(kThreads) col_pass(ColArgs a, const cplx<T>* _
       const int cc = c0 + c;
       if (cc >= a.cols) continue;
       const int col = colmap(cc);
-      int src = e;
-      double fc = 1.0;
-      if (INV) {
-        src = pad_src(e, a.rows_in, L, fc);
-        if (KIND == K_INV && a.prune && src >= 0 && abs(cmode(src, a.rows_in)) > a.Krow) src = -1;
-      }
+      double fc;
+      int src = INV ? pad_src(e, a.rows_in, L, fc) : e;
+      if (KIND == K_INV && a.prune && src >= 0 && abs(cmode(src, a.rows_in)) > a.Krow) src = -1;
       if (src < 0) continue;
-      fac[q] = (T)fc;
-      if constexpr (CURL) {
-        constexpr int D = KIND == K_INV_CURL3 ? 3 : 2;
-        int mi = cmode(src, a.gN0), mj = 0, l;
-        if (D == 3) { mj = cmode(col / a.gnh, a.gN1); l = col % a.gnh; }
-        else l = col;
-        if (a.mask23 && (abs(mi) > a.gN0 / 3 || abs(mj) > a.gN1 / 3 || l > (D == 3 ? a.gN2 : a.gN1) / 3))
+      if constexpr (GRID) {
+        const int mi = cmode(src, a.gN0);
+        int mj = 0, l = col;
+        if (GD == 3) { mj = cmode(col / a.gnh, a.gN1); l = col % a.gnh; }
+        if (a.mask23 && (abs(mi) > a.gN0 / 3 || abs(mj) > a.gN1 / 3 || l > (GD == 3 ? a.gN2 : a.gN1) / 3))
           continue;
         const cplx<T>* p = in + (long long)src * a.in_row + col;
-        const T k0 = (T)(mi * a.ks0);
-        const T k1 = D == 3 ? (T)(mj * a.ks1) : (T)(l * a.ks1);
-        const T k2 = D == 3 ? (T)(l * a.ks2) : (T)0;
-        ok[q] = true;
-        if (f < D) {
+        if constexpr (VEL) {
           ra[q] = p[(long long)f * a.in_field];
-          ka[q] = (T)1;  // marks a plain copy
-          kb[q] = (T)0;
-          continue;
+        } else {
+          // omega_f = i (ka u_ia - kb u_ib)
+          const T k0 = (T)(mi * a.ks0);
+          const T k1 = GD == 3 ? (T)(mj * a.ks1) : (T)(l * a.ks1);
+          const T k2 = GD == 3 ? (T)(l * a.ks2) : (T)0;
+          int ia, ib;
+          if (GD == 2 || f == 2) { ia = 1; ib = 0; ka[q] = k0; kb[q] = k1; }
+          else if (f == 0) { ia = 2; ib = 1; ka[q] = k1; kb[q] = k2; }
+          else { ia = 0; ib = 2; ka[q] = k2; kb[q] = k0; }
+          ra[q] = p[(long long)ia * a.in_field];
+          rb[q] = p[(long long)ib * a.in_field];
         }
-        // omega_f = i (ka * u_ia - kb * u_ib)
-        int ia, ib;
-        if (D == 2) { ia = 1; ib = 0; ka[q] = k0; kb[q] = k1; }
-        else if (f == 3) { ia = 2; ib = 1; ka[q] = k1; kb[q] = k2; }
-        else if (f == 4) { ia = 0; ib = 2; ka[q] = k2; kb[q] = k0; }
-        else { ia = 1; ib = 0; ka[q] = k0; kb[q] = k1; }
-        ra[q] = p[(long long)ia * a.in_field];
-        rb[q] = p[(long long)ib * a.in_field];
       } else {
-        ok[q] = true;
         ra[q] = in[(long long)f * a.in_field + (long long)src * a.in_row + col];
       }
     }
@@ -971,16 +972,14 @@ __global__ void __launch_bounds__(kThreads) col_pass(ColArgs a, const cplx<T>* _
     for (int q = 0; q < CH; ++q) {
       const int idx = threadIdx.x + (base + q) * kThreads;
       if (base + q >= NIT || idx >= L * W) continue;
-      cplx<T> v = mk<T>(0, 0);
-      if (ok[q]) {
-        if constexpr (CURL) {
-          const int f = fg;
-          if (f < (KIND == K_INV_CURL3 ? 3 : 2)) v = ra[q];
-          else v = mul_i<T>(mk<T>(ka[q] * ra[q].x - kb[q] * rb[q].x, ka[q] * ra[q].y - kb[q] * rb[q].y));
-        } else {
-          v = ra[q];
+      cplx<T> v = ra[q];
+      if constexpr (CURL) v = mul_i<T>(mk<T>(ka[q] * ra[q].x - kb[q] * rb[q].x, ka[q] * ra[q].y - kb[q] * rb[q].y));
+      if constexpr (INV) {
+        // the 3/2 rule halves the Nyquist row it copies to +-n/2
+        if (a.rows_in != L) {
+          const int e = idx / W, h = a.rows_in / 2;
+          if (e == h || e == L - h) v = scl<T>(v, (T)0.5);
         }
-        v = scl<T>(v, fac[q]);
       }
       buf[idx] = v;  // idx = e * W + cf
     }
@@ -993,7 +992,7 @@ __global__ void __launch_bounds__(kThreads) col_pass(ColArgs a, const cplx<T>* _
 
   const T s = (T)a.scale;
   if constexpr (INV) {
-    cplx<T>* out = (cplx<T>*)a.out + (long long)b * a.out_member + (long long)fg * a.out_field +
+    cplx<T>* out = (cplx<T>*)a.out + (long long)b * a.out_member + (long long)(a.f_base + fg) * a.out_field +
                    (long long)outer * a.out_outer;
     constexpr int NO = (L * TC + kThreads - 1) / kThreads;
 #pragma unroll
