@@ -51,6 +51,7 @@ struct pf_plan {
   int N[3] = {1, 1, 1};
   int M[3] = {1, 1, 1};
   int nh = 0;
+  long long pitch = 0;           // row pitch of the work buffers (128-byte multiple)
   long long nk = 0, nphys = 0;
   int B = 1;
   bool fp64 = false;
@@ -188,115 +189,210 @@ template <typename T> int row(const pf_plan* p, int L, int mode, const RowArgs& 
 
 ColArgs col_args() {
   ColArgs a{};
-  a.n_outer = 1;
+  a.ng = 1;
+  a.g[0].n_outer = 1;
   a.n_members = 1;
   a.n_fields = 1;
   a.scale = 1.0;
   return a;
 }
 
+// a plain column pass over `cols` unit-stride columns of n_outer matrices
+template <typename T>
+int col_plain(const pf_plan* p, int L, int kind, ColArgs a, int cols, int n_outer, long long in_outer,
+              long long out_outer, hipStream_t s) {
+  ColGeom& g = a.g[0];
+  g.n2c = g.cvalid = g.cols = cols;
+  g.n_outer = n_outer;
+  g.in_outer = in_outer;
+  g.out_outer = out_outer;
+  a.ng = 1;
+  return launch_col<T>(L, kind, a, p->tw.at(L), s);
+}
+
+// Tile width a column pass of this kind uses at length L.
+template <typename T> int tile_width(int L, int kind) {
+  return (kind == K_FWD_STAGE3 || kind == K_FWD_STAGE2) ? col_stage_width<T>(L) : col_tc<T>(L);
+}
+
+// One strided-axis pass over groups of last-axis entries: `ngroups` groups
+// (rows of the axis not transformed) of `nvalid` entries each, group strides
+// in_lp / out_lp.  Tiles cover the first `nvalid` entries rounded up to whole
+// tiles (entries past nvalid are skipped); with `nyq` > 0 the entry l = nyq of
+// every group forms a second, narrow geometry of the same launch, so every
+// main tile is whole and 128-byte aligned.
+template <typename T>
+int col_groups(pf_plan* p, int L, int kind, ColArgs a, int ngroups, long long in_lp, long long out_lp, int nvalid,
+               int nyq, hipStream_t s) {
+  const int tc = tile_width<T>(L, kind);
+  if (tc <= 0) return -4;
+  ColGeom& g = a.g[0];
+  g.n2c = (nvalid + tc - 1) / tc * tc;
+  g.cvalid = nvalid;
+  g.cols = ngroups * g.n2c;
+  g.l0 = 0;
+  g.in_lp = in_lp;
+  g.out_lp = out_lp;
+  g.n_outer = 1;
+  g.in_outer = g.out_outer = 0;
+  g.in_off = g.out_off = 0;
+  a.ng = 1;
+  if (nyq > 0) {
+    ColGeom& n = a.g[1];
+    n = g;
+    n.n2c = n.cvalid = 1;
+    n.cols = ngroups;
+    n.l0 = nyq;
+    n.in_off = n.out_off = nyq;
+    a.ng = 2;
+  }
+  return launch_col<T>(L, kind, a, p->tw.at(L), s);
+}
+
+// The same over the last-axis columns of per-plane matrices (the axis-1
+// passes): `n_outer` matrices of columns l; the Nyquist geometry takes the
+// planes as its groups.
+template <typename T>
+int col_planes(pf_plan* p, int L, int kind, ColArgs a, int n_outer, long long in_outer, long long out_outer,
+               int nvalid, int nyq, hipStream_t s) {
+  const int tc = tile_width<T>(L, kind);
+  if (tc <= 0) return -4;
+  ColGeom& g = a.g[0];
+  g.n2c = (nvalid + tc - 1) / tc * tc;
+  g.cvalid = nvalid;
+  g.cols = g.n2c;
+  g.l0 = 0;
+  g.in_lp = g.out_lp = 0;
+  g.n_outer = n_outer;
+  g.in_outer = in_outer;
+  g.out_outer = out_outer;
+  g.in_off = g.out_off = 0;
+  a.ng = 1;
+  if (nyq > 0) {
+    ColGeom& n = a.g[1];
+    n = g;
+    n.n2c = n.cvalid = 1;
+    n.cols = n_outer;
+    n.in_lp = in_outer;
+    n.out_lp = out_outer;
+    n.n_outer = 1;
+    n.in_outer = n.out_outer = 0;
+    n.l0 = nyq;
+    n.in_off = n.out_off = nyq;
+    a.ng = 2;
+  }
+  return launch_col<T>(L, kind, a, p->tw.at(L), s);
+}
+
 // Inverse passes, the fused last-axis pass and (3D) the axis-1 forward pass
 // of the nonlinear term: u_hat -> input of the axis-0 forward pass (w2),
-// laid out (B, d, M0, [N1,] nh).
+// laid out (B, d, M0, [N1,] P) with the padded row pitch P.
 template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStream_t s) {
   const int B = p->nact;
-  const long long nh = p->nh;
+  const long long nh = p->nh, P = p->pitch;
   const bool pr = p->direct23;
   if (p->dim == 3) {
     const int N0 = p->N[0], N1 = p->N[1], N2 = p->N[2];
     const int M0 = p->M[0], M1 = p->M[1], M2 = p->M[2];
-    const long long A_f = (long long)M0 * N1 * nh, B_f = (long long)M0 * M1 * nh;
-    const long long D_f = (long long)M0 * N1 * nh;
-    // P1: axis-0 inverse with curl, u_hat -> A (6 fields, w1)
+    const long long A_f = (long long)M0 * N1 * P, B_f = (long long)M0 * M1 * P;
+    const long long D_f = (long long)M0 * N1 * nh;  // D keeps the state's row length
+    // retained last-axis entries (2/3 rule) or all but the Nyquist one (3/2 rule)
+    const int lval = pr ? N2 / 3 + 1 : N2 / 2;
+    const int nyq = pr ? 0 : N2 / 2;
+    // P1: axis-0 inverse of u_hat and omega_hat, u_hat -> A (6 fields, w1)
     ColArgs a = col_args();
     a.in = uhat; a.out = p->w1;
-    a.rows_in = N0; a.rows_out = M0; a.cols = N1 * (int)nh;
-    a.in_row = (long long)N1 * nh; a.out_row = (long long)N1 * nh;
+    a.rows_in = N0; a.rows_out = M0;
+    a.in_row = (long long)N1 * nh; a.out_row = (long long)N1 * P;
     a.in_field = p->nk; a.out_field = A_f;
     a.in_member = 3 * p->nk; a.out_member = 6 * A_f;
-    a.n_fields = 6; a.n_members = B;
+    a.n_fields = 3; a.n_members = B;
     a.gN0 = N0; a.gN1 = N1; a.gN2 = N2; a.gnh = (int)nh;
     a.ks0 = p->ks[0]; a.ks1 = p->ks[1]; a.ks2 = p->ks[2];
     a.mask23 = pr;
+    int ngroups = N1;
     if (pr) {
-      a.prune = 1;
+      a.jcompact = 1;
       a.K1 = N1 / 3;
-      a.n2k = N2 / 3 + 1;
-      a.cols = (2 * a.K1 + 1) * a.n2k;
+      ngroups = 2 * a.K1 + 1;
     }
-    a.n_fields = 3;
     {
       PassTimer t(p, PF_PASS_INV0, s);
       a.f_base = 0;
-      PF_TRY(col<T>(p, M0, K_INV_U3, a, s));
+      PF_TRY(col_groups<T>(p, M0, K_INV_U3, a, ngroups, nh, P, lval, nyq, s));
       a.f_base = 3;
-      PF_TRY(col<T>(p, M0, K_INV_CURL3, a, s));
+      PF_TRY(col_groups<T>(p, M0, K_INV_CURL3, a, ngroups, nh, P, lval, nyq, s));
     }
     // P2: axis-1 inverse, A -> Bbuf (w2)
     a = col_args();
     a.in = p->w1; a.out = p->w2;
-    a.rows_in = N1; a.rows_out = M1; a.cols = (int)nh;
-    a.n_outer = M0; a.in_outer = (long long)N1 * nh; a.out_outer = (long long)M1 * nh;
-    a.in_row = nh; a.out_row = nh;
+    a.rows_in = N1; a.rows_out = M1;
+    a.in_row = P; a.out_row = P;
     a.in_field = A_f; a.out_field = B_f;
     a.in_member = 6 * A_f; a.out_member = 6 * B_f;
     a.n_fields = 6; a.n_members = B;
     if (pr) {
       a.prune = 1;
       a.Krow = N1 / 3;
-      a.cols = N2 / 3 + 1;
     }
-    { PassTimer t(p, PF_PASS_INV1, s); PF_TRY(col<T>(p, M1, K_INV, a, s)); }
+    {
+      PassTimer t(p, PF_PASS_INV1, s);
+      PF_TRY(col_planes<T>(p, M1, K_INV, a, M0, (long long)N1 * P, (long long)M1 * P, lval, nyq, s));
+    }
     // P3: last axis c2r, u x omega, r2c, truncate: Bbuf -> C (w1)
     RowArgs r{};
     r.in = p->w2; r.out = p->w1;
     r.n_rows = M0 * M1; r.N = N2; r.nh = (int)nh;
-    r.in_row = nh; r.out_row = nh;
+    r.in_row = P; r.out_row = P;
     r.in_field = B_f; r.out_field = B_f;
     r.in_member = 6 * B_f; r.out_member = 3 * B_f;
     r.n_members = B; r.n_fields = 1;
     r.mask23 = pr; r.scale = 1.0 / M2;
     { PassTimer t(p, PF_PASS_ROW, s); PF_TRY(row<T>(p, M2, ROW_FUSED3, r, s)); }
-    // P4: axis-1 forward + truncation: C -> D (w2)
+    // P4: axis-1 forward + truncation: C -> D (w2), all nh entries.  D has
+    // the state's layout, so the axis-0 forward reads and writes the same
+    // flat column index.
     a = col_args();
     a.in = p->w1; a.out = p->w2;
-    a.rows_in = M1; a.rows_out = N1; a.cols = (int)nh;
-    a.n_outer = M0; a.in_outer = (long long)M1 * nh; a.out_outer = (long long)N1 * nh;
-    a.in_row = nh; a.out_row = nh;
+    a.rows_in = M1; a.rows_out = N1;
+    a.in_row = P; a.out_row = nh;
     a.in_field = B_f; a.out_field = D_f;
     a.in_member = 3 * B_f; a.out_member = 3 * D_f;
     a.n_fields = 3; a.n_members = B; a.scale = 1.0 / M1;
-    { PassTimer t(p, PF_PASS_FWD1, s); PF_TRY(col<T>(p, M1, K_FWD, a, s)); }
+    {
+      PassTimer t(p, PF_PASS_FWD1, s);
+      PF_TRY(col_planes<T>(p, M1, K_FWD, a, M0, (long long)M1 * P, (long long)N1 * nh, (int)nh, 0, s));
+    }
   } else {
     const int N0 = p->N[0], N1 = p->N[1];
     const int M0 = p->M[0], M1 = p->M[1];
-    const long long A_f = (long long)M0 * nh;
-    // P1: axis-0 inverse with the scalar vorticity, u_hat -> A (3 fields, w1)
+    const long long A_f = (long long)M0 * P;
+    const int lval = pr ? N1 / 3 + 1 : (int)nh;
+    // P1: axis-0 inverse of u_hat and the scalar vorticity, u_hat -> A (3 fields, w1)
     ColArgs a = col_args();
     a.in = uhat; a.out = p->w1;
-    a.rows_in = N0; a.rows_out = M0; a.cols = (int)nh;
-    a.in_row = nh; a.out_row = nh;
+    a.rows_in = N0; a.rows_out = M0;
+    a.in_row = nh; a.out_row = P;
     a.in_field = p->nk; a.out_field = A_f;
     a.in_member = 2 * p->nk; a.out_member = 3 * A_f;
-    a.n_fields = 3; a.n_members = B;
+    a.n_members = B;
     a.gN0 = N0; a.gN1 = N1; a.gN2 = N1; a.gnh = (int)nh;
     a.ks0 = p->ks[0]; a.ks1 = p->ks[1];
     a.mask23 = pr;
-    if (pr) a.cols = N1 / 3 + 1;
     {
       PassTimer t(p, PF_PASS_INV0, s);
       a.n_fields = 2;
       a.f_base = 0;
-      PF_TRY(col<T>(p, M0, K_INV_U2, a, s));
+      PF_TRY(col_groups<T>(p, M0, K_INV_U2, a, 1, 0, 0, lval, 0, s));
       a.n_fields = 1;
       a.f_base = 2;
-      PF_TRY(col<T>(p, M0, K_INV_CURL2, a, s));
+      PF_TRY(col_groups<T>(p, M0, K_INV_CURL2, a, 1, 0, 0, lval, 0, s));
     }
     // P2: fused last axis: A -> C (2 fields, w2)
     RowArgs r{};
     r.in = p->w1; r.out = p->w2;
     r.n_rows = M0; r.N = N1; r.nh = (int)nh;
-    r.in_row = nh; r.out_row = nh;
+    r.in_row = P; r.out_row = P;
     r.in_field = A_f; r.out_field = A_f;
     r.in_member = 3 * A_f; r.out_member = 2 * A_f;
     r.n_members = B; r.n_fields = 1;
@@ -306,29 +402,40 @@ template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStrea
   return 0;
 }
 
-// Arguments of the axis-0 forward pass from the front's output (w2).
-ColArgs last_forward_args(const pf_plan* p) {
-  const long long nh = p->nh;
+// The axis-0 forward pass from the front's output (w2) into the state layout:
+// `out` receives the unprojected nonlinear term (K_FWD) or, with the stage
+// kinds, the stage update is applied to u_hat (out unused).
+template <typename T> int last_forward(pf_plan* p, int kind, void* out, const StageArgs* st, hipStream_t s) {
+  const long long nh = p->nh, P = p->pitch;
   const int d = p->dim;
   const int M0 = p->M[0], N0 = p->N[0];
-  const long long inner = d == 3 ? (long long)p->N[1] * nh : nh;
-  const long long D_f = (long long)M0 * inner;
   ColArgs a = col_args();
-  a.in = p->w2; a.out = p->w1;
-  a.rows_in = M0; a.rows_out = N0; a.cols = (int)inner;
-  a.in_row = inner; a.out_row = inner;
-  a.in_field = D_f; a.out_field = p->nk;
-  a.in_member = d * D_f; a.out_member = d * p->nk;
+  a.in = p->w2; a.out = out;
+  a.rows_in = M0; a.rows_out = N0;
   a.n_fields = d; a.n_members = p->nact; a.scale = 1.0 / M0;
-  return a;
+  a.out_field = p->nk; a.out_member = d * p->nk;
+  if (st) a.st = *st;
+  if (d == 3) {
+    // D and the state share one layout: one flat group of N1 * nh columns
+    const int N1 = p->N[1];
+    const long long D_f = (long long)M0 * N1 * nh;
+    a.in_row = a.out_row = (long long)N1 * nh;
+    a.in_field = D_f; a.in_member = d * D_f;
+    a.gN1 = N1; a.gnh = (int)nh;
+    return col_groups<T>(p, M0, kind, a, 1, 0, 0, N1 * (int)nh, 0, s);
+  }
+  const long long D_f = (long long)M0 * P;
+  a.in_row = P; a.out_row = nh;
+  a.in_field = D_f; a.in_member = d * D_f;
+  a.gN1 = 1; a.gnh = (int)nh;
+  return col_groups<T>(p, M0, kind, a, 1, 0, 0, (int)nh, 0, s);
 }
 
 // Nonlinear term (unprojected) of u_hat into the plan's R buffer (w1).
 template <typename T> int nonlinear_unprojected(pf_plan* p, const void* uhat, hipStream_t s) {
   int rc = nonlinear_front<T>(p, uhat, s);
   if (rc) return rc;
-  const ColArgs a = last_forward_args(p);
-  { PassTimer t(p, PF_PASS_FWD0, s); PF_TRY(col<T>(p, p->M[0], K_FWD, a, s)); }
+  { PassTimer t(p, PF_PASS_FWD0, s); PF_TRY(last_forward<T>(p, K_FWD, p->w1, nullptr, s)); }
   return 0;
 }
 
@@ -340,19 +447,18 @@ template <typename T> int backward_fields(pf_plan* p, const void* in, void* out,
     const int N0 = p->N[0], N1 = p->N[1], N2 = p->N[2];
     ColArgs a = col_args();
     a.in = in; a.out = p->w1;
-    a.rows_in = N0; a.rows_out = N0; a.cols = N1 * (int)nh;
+    a.rows_in = N0; a.rows_out = N0;
     a.in_row = (long long)N1 * nh; a.out_row = a.in_row;
     a.in_field = p->nk; a.out_field = p->nk;
     a.n_fields = nf;
-    PF_TRY(col<T>(p, N0, K_INV, a, s));
+    PF_TRY(col_plain<T>(p, N0, K_INV, a, N1 * (int)nh, 1, 0, 0, s));
     a = col_args();
     a.in = p->w1; a.out = p->w1;
-    a.rows_in = N1; a.rows_out = N1; a.cols = (int)nh;
-    a.n_outer = N0; a.in_outer = (long long)N1 * nh; a.out_outer = a.in_outer;
+    a.rows_in = N1; a.rows_out = N1;
     a.in_row = nh; a.out_row = nh;
     a.in_field = p->nk; a.out_field = p->nk;
     a.n_fields = nf;
-    PF_TRY(col<T>(p, N1, K_INV, a, s));
+    PF_TRY(col_plain<T>(p, N1, K_INV, a, (int)nh, N0, (long long)N1 * nh, (long long)N1 * nh, s));
     RowArgs r{};
     r.in = p->w1; r.out = out;
     r.n_rows = N0 * N1; r.N = N2; r.nh = (int)nh;
@@ -364,11 +470,11 @@ template <typename T> int backward_fields(pf_plan* p, const void* in, void* out,
     const int N0 = p->N[0], N1 = p->N[1];
     ColArgs a = col_args();
     a.in = in; a.out = p->w1;
-    a.rows_in = N0; a.rows_out = N0; a.cols = (int)nh;
+    a.rows_in = N0; a.rows_out = N0;
     a.in_row = nh; a.out_row = nh;
     a.in_field = p->nk; a.out_field = p->nk;
     a.n_fields = nf;
-    PF_TRY(col<T>(p, N0, K_INV, a, s));
+    PF_TRY(col_plain<T>(p, N0, K_INV, a, (int)nh, 1, 0, 0, s));
     RowArgs r{};
     r.in = p->w1; r.out = out;
     r.n_rows = N0; r.N = N1; r.nh = (int)nh;
@@ -393,19 +499,18 @@ template <typename T> int forward_fields(pf_plan* p, const void* in, void* out, 
     PF_TRY(row<T>(p, N2, ROW_R2C, r, s));
     ColArgs a = col_args();
     a.in = out; a.out = out;
-    a.rows_in = N1; a.rows_out = N1; a.cols = (int)nh;
-    a.n_outer = N0; a.in_outer = (long long)N1 * nh; a.out_outer = a.in_outer;
+    a.rows_in = N1; a.rows_out = N1;
     a.in_row = nh; a.out_row = nh;
     a.in_field = p->nk; a.out_field = p->nk;
     a.n_fields = nf; a.scale = 1.0 / N1;
-    PF_TRY(col<T>(p, N1, K_FWD, a, s));
+    PF_TRY(col_plain<T>(p, N1, K_FWD, a, (int)nh, N0, (long long)N1 * nh, (long long)N1 * nh, s));
     a = col_args();
     a.in = out; a.out = out;
-    a.rows_in = N0; a.rows_out = N0; a.cols = N1 * (int)nh;
+    a.rows_in = N0; a.rows_out = N0;
     a.in_row = (long long)N1 * nh; a.out_row = a.in_row;
     a.in_field = p->nk; a.out_field = p->nk;
     a.n_fields = nf; a.scale = 1.0 / N0;
-    PF_TRY(col<T>(p, N0, K_FWD, a, s));
+    PF_TRY(col_plain<T>(p, N0, K_FWD, a, N1 * (int)nh, 1, 0, 0, s));
   } else {
     const int N0 = p->N[0], N1 = p->N[1];
     RowArgs r{};
@@ -417,11 +522,11 @@ template <typename T> int forward_fields(pf_plan* p, const void* in, void* out, 
     PF_TRY(row<T>(p, N1, ROW_R2C, r, s));
     ColArgs a = col_args();
     a.in = out; a.out = out;
-    a.rows_in = N0; a.rows_out = N0; a.cols = (int)nh;
+    a.rows_in = N0; a.rows_out = N0;
     a.in_row = nh; a.out_row = nh;
     a.in_field = p->nk; a.out_field = p->nk;
     a.n_fields = nf; a.scale = 1.0 / N0;
-    PF_TRY(col<T>(p, N0, K_FWD, a, s));
+    PF_TRY(col_plain<T>(p, N0, K_FWD, a, (int)nh, 1, 0, 0, s));
   }
   return 0;
 }
@@ -524,11 +629,9 @@ template <typename T> int step_impl(pf_plan* p, void* uhat, long long n_steps, c
       if (p->d.nonlinear && fused) {
         int rc = nonlinear_front<T>(p, uhat, s);
         if (rc) return rc;
-        ColArgs c = last_forward_args(p);
         a.rhs_nl = nullptr;
-        c.st = a;
         PassTimer t(p, PF_PASS_FWD0_STAGE, s);
-        PF_TRY(col<T>(p, p->M[0], p->dim == 3 ? K_FWD_STAGE3 : K_FWD_STAGE2, c, s));
+        PF_TRY(last_forward<T>(p, p->dim == 3 ? K_FWD_STAGE3 : K_FWD_STAGE2, nullptr, &a, s));
       } else {
         if (p->d.nonlinear) {
           int rc = nonlinear_unprojected<T>(p, uhat, s);
@@ -614,17 +717,20 @@ int pf_plan_create(const pf_desc* desc, pf_plan** out) {
   p->nc = d.noise_type == PF_NOISE_THERMAL ? d.dim * d.dim : d.dim;
 
   const long long B = p->B, nh = p->nh;
+  const long long align = 128 / (long long)p->csz;
+  p->pitch = (nh + align - 1) / align * align;
+  const long long P = p->pitch;
   long long w1c = B * d.dim * p->nk, w2c = 0;  // complex elements
   if (d.nonlinear) {
     if (d.dim == 3) {
       const long long M0 = p->M[0], M1 = p->M[1], N0 = p->N[0], N1 = p->N[1];
-      const long long A = B * 6 * M0 * N1 * nh, Bb = B * 6 * M0 * M1 * nh, C = B * 3 * M0 * M1 * nh;
+      const long long A = B * 6 * M0 * N1 * P, Bb = B * 6 * M0 * M1 * P, C = B * 3 * M0 * M1 * P;
       const long long D = B * 3 * M0 * N1 * nh, R = B * 3 * N0 * N1 * nh;
       w1c = std::max(std::max(w1c, A), std::max(C, R));
       w2c = std::max(Bb, D);
     } else {
       const long long M0 = p->M[0], N0 = p->N[0];
-      const long long A = B * 3 * M0 * nh, C = B * 2 * M0 * nh, R = B * 2 * N0 * nh;
+      const long long A = B * 3 * M0 * P, C = B * 2 * M0 * P, R = B * 2 * N0 * nh;
       w1c = std::max(w1c, std::max(A, R));
       w2c = C;
     }

@@ -2,13 +2,16 @@
 //
 // HBM layout (C order, complex interleaved, one block per member):
 //   state u_hat         (B, d, N0, [N1,] nh)      nh = N_last/2 + 1
-//   3D nonlinear passes (per member, F = 6 for u_hat and omega_hat):
-//     A  (F, M0, N1, nh)   axis-0 inverse, padded       <- col_pass CURL3
-//     B  (F, M0, M1, nh)   axis-1 inverse, padded       <- col_pass
-//     C  (3, M0, M1, nh)   c2r, u x omega, r2c, cut     <- row_pass FUSED3
+//   3D nonlinear passes (per member, F = 6 for u_hat and omega_hat; rows of
+//   A, B, C padded to the pitch P >= nh, a multiple of 128 bytes):
+//     A  (F, M0, N1, P)    axis-0 inverse, padded       <- col_pass U3 + CURL3
+//     B  (F, M0, M1, P)    axis-1 inverse, padded       <- col_pass
+//     C  (3, M0, M1, P)    c2r, u x omega, r2c, cut     <- row_pass FUSED3
 //     D  (3, M0, N1, nh)   axis-1 forward, truncated    <- col_pass
-//     R  (3, N0, N1, nh)   axis-0 forward, truncated    <- col_pass
-//   2D: A (3, M0, nh) -> C (2, M0, nh) -> R (2, N0, nh)
+//     R  (3, N0, N1, nh)   axis-0 forward, truncated    <- col_pass (or fused stage)
+//   2D: A (3, M0, P) -> C (2, M0, P) -> R (2, N0, nh)
+// D has the state's layout so that the axis-0 forward pass walks one flat
+// column index over both.
 // The padded physical fields never reach HBM: the last axis is taken to
 // physical space, multiplied and brought back inside one block.
 #pragma once
@@ -830,15 +833,28 @@ enum ColKind {
   K_INV_U2 = 7,      // velocity components of a 2D field (2/3 mask)
 };
 
+// Column enumeration of one launch geometry: column cc = jj * n2c + l of
+// the tile space sits at in_off + jj * in_lp + l in the input and
+// out_off + jj * out_lp + l in the output; columns with l >= cvalid are
+// neither loaded nor stored; l0 is the last-axis index of l = 0.
+struct ColGeom {
+  int cols;                     // enumerated columns per matrix
+  int n2c, cvalid, l0;
+  int n_outer;                  // matrices per field
+  int n_tiles;                  // column tiles per matrix (set by the launcher)
+  long long in_lp, out_lp;
+  long long in_outer, out_outer;
+  long long in_off, out_off;
+  long long n_blocks;           // blocks of this geometry (set by the launcher)
+};
+
 struct ColArgs {
   const void* in;
   void* out;
   int rows_in, rows_out;        // rows read / written per matrix
-  int cols;                     // columns per matrix (compact count when pruned)
-  int n_tiles;                  // column tiles per matrix (set by the launcher)
-  int n_outer;                  // matrices per field
+  ColGeom g[2];                 // g[1]: the narrow Nyquist geometry (ng = 2), scheduled first
+  int ng;
   long long in_row, out_row;    // element strides between rows
-  long long in_outer, out_outer;
   long long in_field, out_field;
   long long in_member, out_member;
   int n_fields;                 // fields per member
@@ -849,7 +865,7 @@ struct ColArgs {
   int mask23;                   // 2/3 mask while loading (curl kinds)
   int prune;                    // visit retained modes only (2/3 rule)
   int Krow;                     // pruned inverse: rows with |k| > Krow are zero
-  int K1, n2k;                  // pruned 3D axis-0: |k_j| <= K1, l < n2k
+  int jcompact, K1;             // group jj -> retained axis-1 index |k_j| <= K1 (2/3 rule)
   int f_base;                   // first output field of the launch
   StageArgs st;                 // fused stage kinds
 };
@@ -897,70 +913,94 @@ __global__ void __launch_bounds__(kThreads) col_pass(ColArgs a, const cplx<T>* _
   constexpr int W = NFLD * TC;  // LDS columns
   constexpr int NIT = (L * W + kThreads - 1) / kThreads;
   constexpr int CH = NIT < 16 ? NIT : 16;  // loads kept in flight per thread
+  static_assert(kThreads % TC == 0, "tile width must divide the block size");
   __shared__ cplx<T> tws[tw_alloc(L)];
   __shared__ cplx<T> buf[L * W];
   for (int m = threadIdx.x; m < tw_size(L); m += kThreads) tws[m] = tw[m];
 
   const int nfg = STAGE ? 1 : a.n_fields;
   long long lg = xcd_order(blockIdx.x, gridDim.x);
+  const bool nyqg = a.ng > 1 && lg < a.g[1].n_blocks;
+  const ColGeom gm = nyqg ? a.g[1] : a.g[0];
+  if (!nyqg && a.ng > 1) lg -= a.g[1].n_blocks;
   const int fg = (int)(lg % nfg);
   lg /= nfg;
-  const int tile = (int)(lg % a.n_tiles);
-  lg /= a.n_tiles;
-  const int outer = (int)(lg % a.n_outer);
-  const int b = (int)(lg / a.n_outer);
+  const int tile = (int)(lg % gm.n_tiles);
+  lg /= gm.n_tiles;
+  const int outer = (int)(lg % gm.n_outer);
+  const int b = (int)(lg / gm.n_outer);
   const int c0 = tile * TC;
-  const cplx<T>* in = (const cplx<T>*)a.in + (long long)b * a.in_member + (long long)outer * a.in_outer;
+  const cplx<T>* in = (const cplx<T>*)a.in + gm.in_off + (long long)b * a.in_member + (long long)outer * gm.in_outer;
 
-  // compact column -> column of the matrix
-  auto colmap = [&](int cc) -> int {
-    if ((KIND == K_INV_CURL3 || KIND == K_INV_U3) && a.prune) {
-      const int jj = cc / a.n2k, l = cc % a.n2k;
-      const int j = jj <= a.K1 ? jj : jj + a.gN1 - (2 * a.K1 + 1);
-      return j * a.gnh + l;
-    }
-    return cc;
+  // enumerated column -> (group j, column l within the group)
+  auto cpos = [&](int cc, int& j, int& l) {
+    int jj;
+    if (gm.cols <= gm.n2c) { jj = 0; l = cc; }          // one group
+    else if (gm.n2c == 1) { jj = cc; l = 0; }           // one column per group
+    else { jj = cc / gm.n2c; l = cc % gm.n2c; }
+    j = (a.jcompact && jj > a.K1) ? jj + a.gN1 - (2 * a.K1 + 1) : jj;
   };
+  // LDS column cf -> field, tile column, (j, l), and whether it is stored
+  struct Slot { int f, c, j, l; bool ok; };
+  auto slot = [&](int cf) {
+    Slot r;
+    r.f = STAGE ? cf / TC : fg;
+    r.c = STAGE ? cf % TC : cf;
+    const int cc = c0 + r.c;
+    r.ok = cc < gm.cols;
+    cpos(r.ok ? cc : 0, r.j, r.l);
+    r.ok = r.ok && r.l < gm.cvalid;
+    return r;
+  };
+  // With W dividing the block size every thread touches one LDS column.
+  constexpr bool HOIST = kThreads % W == 0;
+  Slot ts{};
+  if constexpr (HOIST) ts = slot(threadIdx.x % W);
 
   // ---- load: per chunk, issue every load, then combine and store to LDS
+  // wavenumbers (ka, kb) of the curl omega_f = i (ka u_ia - kb u_ib) at row
+  // e, column (j, l) of the unpadded grid
+  auto curl_k = [&](int f, int src, int j, int l, int& ia, int& ib, T& ka, T& kb) {
+    const int mi = cmode(src, a.gN0);
+    const int mj = GD == 3 ? cmode(j, a.gN1) : 0;
+    const T k0 = (T)(mi * a.ks0);
+    const T k1 = GD == 3 ? (T)(mj * a.ks1) : (T)(l * a.ks1);
+    const T k2 = GD == 3 ? (T)(l * a.ks2) : (T)0;
+    if (GD == 2 || f == 2) { ia = 1; ib = 0; ka = k0; kb = k1; }
+    else if (f == 0) { ia = 2; ib = 1; ka = k1; kb = k2; }
+    else { ia = 0; ib = 2; ka = k2; kb = k0; }
+  };
   for (int base = 0; base < NIT; base += CH) {
     cplx<T> ra[CH], rb[CH];
-    T ka[CH], kb[CH];
 #pragma unroll
     for (int q = 0; q < CH; ++q) {
       const int idx = threadIdx.x + (base + q) * kThreads;
       ra[q] = mk<T>(0, 0);
       rb[q] = mk<T>(0, 0);
-      ka[q] = kb[q] = (T)0;
       if (base + q >= NIT || idx >= L * W) continue;
-      const int e = idx / W, cf = idx % W;
-      const int f = STAGE ? cf / TC : fg;
-      const int c = STAGE ? cf % TC : cf;
-      const int cc = c0 + c;
-      if (cc >= a.cols) continue;
-      const int col = colmap(cc);
+      const int e = idx / W;
+      const Slot sl = HOIST ? ts : slot(idx % W);
+      if (!sl.ok) continue;
+      const int f = sl.f, j = sl.j;
+      int l = sl.l;
+      const long long col = (long long)j * gm.in_lp + l;
       double fc;
       int src = INV ? pad_src(e, a.rows_in, L, fc) : e;
       if (KIND == K_INV && a.prune && src >= 0 && abs(cmode(src, a.rows_in)) > a.Krow) src = -1;
       if (src < 0) continue;
       if constexpr (GRID) {
         const int mi = cmode(src, a.gN0);
-        int mj = 0, l = col;
-        if (GD == 3) { mj = cmode(col / a.gnh, a.gN1); l = col % a.gnh; }
+        const int mj = GD == 3 ? cmode(j, a.gN1) : 0;
+        l += gm.l0;
         if (a.mask23 && (abs(mi) > a.gN0 / 3 || abs(mj) > a.gN1 / 3 || l > (GD == 3 ? a.gN2 : a.gN1) / 3))
           continue;
         const cplx<T>* p = in + (long long)src * a.in_row + col;
         if constexpr (VEL) {
           ra[q] = p[(long long)f * a.in_field];
         } else {
-          // omega_f = i (ka u_ia - kb u_ib)
-          const T k0 = (T)(mi * a.ks0);
-          const T k1 = GD == 3 ? (T)(mj * a.ks1) : (T)(l * a.ks1);
-          const T k2 = GD == 3 ? (T)(l * a.ks2) : (T)0;
           int ia, ib;
-          if (GD == 2 || f == 2) { ia = 1; ib = 0; ka[q] = k0; kb[q] = k1; }
-          else if (f == 0) { ia = 2; ib = 1; ka[q] = k1; kb[q] = k2; }
-          else { ia = 0; ib = 2; ka[q] = k2; kb[q] = k0; }
+          T ka, kb;
+          curl_k(f, src, j, l, ia, ib, ka, kb);
           ra[q] = p[(long long)ia * a.in_field];
           rb[q] = p[(long long)ib * a.in_field];
         }
@@ -973,7 +1013,17 @@ __global__ void __launch_bounds__(kThreads) col_pass(ColArgs a, const cplx<T>* _
       const int idx = threadIdx.x + (base + q) * kThreads;
       if (base + q >= NIT || idx >= L * W) continue;
       cplx<T> v = ra[q];
-      if constexpr (CURL) v = mul_i<T>(mk<T>(ka[q] * ra[q].x - kb[q] * rb[q].x, ka[q] * ra[q].y - kb[q] * rb[q].y));
+      if constexpr (CURL) {
+        // entries not loaded are zero in ra and rb, so any (ka, kb) serves there
+        const int e = idx / W;
+        const Slot sl = HOIST ? ts : slot(idx % W);
+        double fc;
+        int src = pad_src(e, a.rows_in, L, fc);
+        int ia, ib;
+        T ka, kb;
+        curl_k(fg, src < 0 ? 0 : src, sl.j, sl.l + gm.l0, ia, ib, ka, kb);
+        v = mul_i<T>(mk<T>(ka * ra[q].x - kb * rb[q].x, ka * ra[q].y - kb * rb[q].y));
+      }
       if constexpr (INV) {
         // the 3/2 rule halves the Nyquist row it copies to +-n/2
         if (a.rows_in != L) {
@@ -992,16 +1042,18 @@ __global__ void __launch_bounds__(kThreads) col_pass(ColArgs a, const cplx<T>* _
 
   const T s = (T)a.scale;
   if constexpr (INV) {
-    cplx<T>* out = (cplx<T>*)a.out + (long long)b * a.out_member + (long long)(a.f_base + fg) * a.out_field +
-                   (long long)outer * a.out_outer;
+    cplx<T>* out = (cplx<T>*)a.out + gm.out_off + (long long)b * a.out_member + (long long)(a.f_base + fg) * a.out_field +
+                   (long long)outer * gm.out_outer;
     constexpr int NO = (L * TC + kThreads - 1) / kThreads;
+    // TC divides the block size: one tile column per thread
+    const Slot os = slot(threadIdx.x % TC);
+    const long long ocol = (long long)os.j * gm.out_lp + os.l;
 #pragma unroll
     for (int q = 0; q < NO; ++q) {
       const int idx = threadIdx.x + q * kThreads;
-      if (idx >= L * TC) continue;
-      const int e = idx / TC, c = idx % TC;
-      const int cc = c0 + c;
-      if (cc < a.cols) out[(long long)e * a.out_row + colmap(cc)] = scl<T>(buf[e * W + c], s);
+      if (idx >= L * TC || !os.ok) continue;
+      const int e = idx / TC;
+      out[(long long)e * a.out_row + ocol] = scl<T>(buf[e * W + os.c], s);
     }
   } else {
     const int n = a.rows_out, h = n / 2;
@@ -1013,35 +1065,45 @@ __global__ void __launch_bounds__(kThreads) col_pass(ColArgs a, const cplx<T>* _
       return scl<T>(v, s);
     };
     if constexpr (STAGE) {
-      // modes of the tile: fetch u_hat and u0 of SCH modes, then update them
+      // modes of the tile: fetch u_hat and u0 of SCH modes, then update them.
+      // State index of (row o, group j, column l): (o * N1 + j) * nh + l0 + l
+      // (2D: o * nh + l0 + l).
       constexpr int SCH = 4;
+      const long long rowlen = NFLD == 3 ? (long long)a.gN1 * a.gnh : (long long)a.gnh;
+      // TC divides the block size: one tile column per thread
+      const int c = threadIdx.x % TC;
+      int j, l;
+      const bool ok = c0 + c < gm.cols;
+      cpos(ok ? c0 + c : 0, j, l);
+      const bool store = ok && l < gm.cvalid;
+      const long long mcol = (long long)j * a.gnh + gm.l0 + l;
       for (int idx0 = threadIdx.x; idx0 < n * TC; idx0 += SCH * kThreads) {
         cplx<T> uu[SCH][NFLD], zz[SCH][NFLD];
 #pragma unroll
         for (int q = 0; q < SCH; ++q) {
           const int idx = idx0 + q * kThreads;
-          const int o = idx / TC, cc = c0 + idx % TC;
-          if (idx < n * TC && cc < a.cols) stage_fetch<T, NFLD>(a.st, (long long)o * a.cols + cc, b, uu[q], zz[q]);
+          if (idx < n * TC && store) stage_fetch<T, NFLD>(a.st, (long long)(idx / TC) * rowlen + mcol, b, uu[q], zz[q]);
         }
 #pragma unroll
         for (int q = 0; q < SCH; ++q) {
           const int idx = idx0 + q * kThreads;
-          const int o = idx / TC, c = idx % TC;
-          const int cc = c0 + c;
-          if (idx >= n * TC || cc >= a.cols) continue;
+          if (idx >= n * TC || !store) continue;
+          const int o = idx / TC;
           cplx<T> nl[NFLD];
 #pragma unroll
           for (int f = 0; f < NFLD; ++f) nl[f] = trunc(o, f * TC + c);
-          stage_point<T, NFLD>(a.st, (long long)o * a.cols + cc, b, nl, uu[q], zz[q]);
+          stage_point<T, NFLD>(a.st, (long long)o * rowlen + mcol, b, nl, uu[q], zz[q]);
         }
       }
     } else {
-      cplx<T>* out = (cplx<T>*)a.out + (long long)b * a.out_member + (long long)fg * a.out_field +
-                     (long long)outer * a.out_outer;
+      cplx<T>* out = (cplx<T>*)a.out + gm.out_off + (long long)b * a.out_member + (long long)fg * a.out_field +
+                     (long long)outer * gm.out_outer;
+      const Slot os = slot(threadIdx.x % TC);
+      const long long ocol = (long long)os.j * gm.out_lp + os.l;
       for (int idx = threadIdx.x; idx < n * TC; idx += kThreads) {
-        const int o = idx / TC, c = idx % TC;
-        const int cc = c0 + c;
-        if (cc < a.cols) out[(long long)o * a.out_row + cc] = trunc(o, c);
+        if (!os.ok) break;
+        const int o = idx / TC;
+        out[(long long)o * a.out_row + ocol] = trunc(o, os.c);
       }
     }
   }
