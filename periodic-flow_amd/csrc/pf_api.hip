@@ -226,6 +226,9 @@ int col_groups(pf_plan* p, int L, int kind, ColArgs a, int ngroups, long long in
                int nyq, hipStream_t s) {
   const int tc = tile_width<T>(L, kind);
   if (tc <= 0) return -4;
+  // the fused stage kinds address the state by one flat column index
+  if ((kind == K_FWD_STAGE3 || kind == K_FWD_STAGE2) && (ngroups != 1 || nyq > 0))
+    return fail(PF_EINVAL, "stage pass needs one flat column group");
   ColGeom& g = a.g[0];
   g.n2c = (nvalid + tc - 1) / tc * tc;
   g.cvalid = nvalid;

@@ -382,28 +382,45 @@ template <typename T, int D> struct Mode {
   bool nyq;
 };
 
+// Wavenumbers of the mode at grid indices (i, j, l) (2D: (i, l), j unused).
 template <typename T, int D>
-__device__ __forceinline__ Mode<T, D> decode(const StageArgs& a, long long idx) {
+__device__ __forceinline__ Mode<T, D> mode_at(const StageArgs& a, int i, int j, int l) {
   Mode<T, D> m;
-  const int l = (int)(idx % a.nh);
-  const long long t = idx / a.nh;
   if constexpr (D == 3) {
-    const int j = (int)(t % a.N1), i = (int)(t / a.N1);
     m.k[0] = (T)(cmode(i, a.N0) * a.ks0);
     m.k[1] = (T)(cmode(j, a.N1) * a.ks1);
     m.k[2] = (T)(l * a.ks2);
     m.nyq = (a.N0 % 2 == 0 && i == a.N0 / 2) || (a.N1 % 2 == 0 && j == a.N1 / 2) ||
             (a.N2 % 2 == 0 && l == a.N2 / 2);
   } else {
-    const int i = (int)t;
     m.k[0] = (T)(cmode(i, a.N0) * a.ks0);
     m.k[1] = (T)(l * a.ks1);
     m.nyq = (a.N0 % 2 == 0 && i == a.N0 / 2) || (a.N1 % 2 == 0 && l == a.N1 / 2);
   }
   m.k2 = 0;
 #pragma unroll
-  for (int i = 0; i < D; ++i) m.k2 += m.k[i] * m.k[i];
+  for (int q = 0; q < D; ++q) m.k2 += m.k[q] * m.k[q];
   return m;
+}
+
+// Mode of flat index idx = (i * N1 + j) * nh + l (2D: i * nh + l); 32-bit
+// division whenever a member's spectrum has fewer than 2^32 modes.
+template <typename T, int D>
+__device__ __forceinline__ Mode<T, D> decode(const StageArgs& a, long long idx) {
+  int i, j = 0, l;
+  if (a.nk < (1ll << 32)) {
+    const unsigned u = (unsigned)idx, nh = (unsigned)a.nh;
+    const unsigned t = u / nh;
+    l = (int)(u - t * nh);
+    if constexpr (D == 3) { i = (int)(t / (unsigned)a.N1); j = (int)(t - (unsigned)i * (unsigned)a.N1); }
+    else i = (int)t;
+  } else {
+    const long long t = idx / a.nh;
+    l = (int)(idx - t * a.nh);
+    if constexpr (D == 3) { i = (int)(t / a.N1); j = (int)(t - (long long)i * a.N1); }
+    else i = (int)t;
+  }
+  return mode_at<T, D>(a, i, j, l);
 }
 
 template <typename T, int D> __device__ __forceinline__ void project(cplx<T>* v, const Mode<T, D>& m) {
@@ -419,10 +436,12 @@ template <typename T, int D> __device__ __forceinline__ void project(cplx<T>* v,
 // nlin: the unprojected nonlinear term of this mode when the caller already
 // holds it (fused forward pass), else it is read from a.rhs_nl.
 // upre / uzpre: u_hat and u0 of this mode already fetched (stage_fetch).
+// mpre: the mode's wavenumbers when the caller knows its grid indices.
 template <typename T, int D>
 __device__ __forceinline__ void stage_point(const StageArgs& a, long long idx, int b, const cplx<T>* nlin,
-                                            const cplx<T>* upre = nullptr, const cplx<T>* uzpre = nullptr) {
-  const Mode<T, D> m = decode<T, D>(a, idx);
+                                            const cplx<T>* upre = nullptr, const cplx<T>* uzpre = nullptr,
+                                            const Mode<T, D>* mpre = nullptr) {
+  const Mode<T, D> m = mpre ? *mpre : decode<T, D>(a, idx);
   const long long mb = (long long)b * D * a.nk;
   cplx<T>* uh = (cplx<T>*)a.uhat + mb + idx;
   cplx<T>* u0p = (cplx<T>*)a.u0 + mb + idx;
@@ -947,6 +966,13 @@ __global__ void __launch_bounds__(kThreads) col_pass(ColArgs a, const cplx<T>* _
     r.f = STAGE ? cf / TC : fg;
     r.c = STAGE ? cf % TC : cf;
     const int cc = c0 + r.c;
+    if constexpr (STAGE) {
+      // the stage kinds run over one flat group (checked by the launcher)
+      r.j = 0;
+      r.l = cc;
+      r.ok = cc < gm.cvalid;
+      return r;
+    }
     r.ok = cc < gm.cols;
     cpos(r.ok ? cc : 0, r.j, r.l);
     r.ok = r.ok && r.l < gm.cvalid;
@@ -1072,11 +1098,11 @@ __global__ void __launch_bounds__(kThreads) col_pass(ColArgs a, const cplx<T>* _
       const long long rowlen = NFLD == 3 ? (long long)a.gN1 * a.gnh : (long long)a.gnh;
       // TC divides the block size: one tile column per thread
       const int c = threadIdx.x % TC;
-      int j, l;
-      const bool ok = c0 + c < gm.cols;
-      cpos(ok ? c0 + c : 0, j, l);
-      const bool store = ok && l < gm.cvalid;
-      const long long mcol = (long long)j * a.gnh + gm.l0 + l;
+      const bool store = c0 + c < gm.cvalid;
+      const long long mcol = gm.l0 + c0 + c;
+      // grid indices (j, l) of the column: one division per thread
+      const int jt = NFLD == 3 ? (int)((unsigned)mcol / (unsigned)a.gnh) : 0;
+      const int lt = (int)mcol - jt * a.gnh;
       for (int idx0 = threadIdx.x; idx0 < n * TC; idx0 += SCH * kThreads) {
         cplx<T> uu[SCH][NFLD], zz[SCH][NFLD];
 #pragma unroll
@@ -1092,7 +1118,8 @@ __global__ void __launch_bounds__(kThreads) col_pass(ColArgs a, const cplx<T>* _
           cplx<T> nl[NFLD];
 #pragma unroll
           for (int f = 0; f < NFLD; ++f) nl[f] = trunc(o, f * TC + c);
-          stage_point<T, NFLD>(a.st, (long long)o * rowlen + mcol, b, nl, uu[q], zz[q]);
+          const Mode<T, NFLD> md = mode_at<T, NFLD>(a.st, o, jt, lt);
+          stage_point<T, NFLD>(a.st, (long long)o * rowlen + mcol, b, nl, uu[q], zz[q], &md);
         }
       }
     } else {
