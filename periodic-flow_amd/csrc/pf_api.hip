@@ -629,6 +629,7 @@ template <typename T> int step_impl(pf_plan* p, void* uhat, long long n_steps, c
       }
       StageArgs a = stage_args(p, uhat, stage, nz, noise_on);
       if (stage == first && first > 0) a.save_u0 = 2;  // u_hat is still u0; nothing to save
+      a.single_draw = first == ns - 1;                   // this step's draws feed this stage only
       if (p->d.nonlinear && fused) {
         int rc = nonlinear_front<T>(p, uhat, s);
         if (rc) return rc;

@@ -296,18 +296,23 @@ __device__ __forceinline__ U4 philox(U4 c, uint32_t k0, uint32_t k1) {
   return c;
 }
 
-__device__ __forceinline__ void sc_pi(float x, float* s, float* c) { sincospif(x, s, c); }
-__device__ __forceinline__ void sc_pi(double x, double* s, double* c) { sincospi(x, s, c); }
-__device__ __forceinline__ float ln(float x) { return logf(x); }
-__device__ __forceinline__ double ln(double x) { return log(x); }
-
-template <typename T> __device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, T& g0, T& g1) {
-  // u1 in (0, 1], u2 in [0, 1)
-  const T u1 = ((T)a + (T)0.5) * (T)2.3283064365386963e-10;
-  const T u2 = (T)b * (T)2.3283064365386963e-10;
-  const T r = sqrt((T)-2 * ln(u1));
-  T sn, cs;
-  sc_pi((T)2 * u2, &sn, &cs);
+// u1 in (0, 1], u2 in [0, 1]: g0 = r cos(2 pi u2), g1 = r sin(2 pi u2),
+// r = sqrt(-2 ln u1).  FP32 uses the hardware log2 and the hardware sine
+// and cosine, whose argument is in revolutions (v_sin_f32(x) = sin(2 pi x)),
+// so the angle needs no range reduction; FP64 uses the library functions.
+__device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, float& g0, float& g1) {
+  const float u1 = ((float)a + 0.5f) * 2.3283064365386963e-10f;
+  const float u2 = (float)b * 2.3283064365386963e-10f;
+  const float r = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));  // -2 ln 2 log2 u1
+  g0 = r * __builtin_amdgcn_cosf(u2);
+  g1 = r * __builtin_amdgcn_sinf(u2);
+}
+__device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, double& g0, double& g1) {
+  const double u1 = ((double)a + 0.5) * 2.3283064365386963e-10;
+  const double u2 = (double)b * 2.3283064365386963e-10;
+  const double r = sqrt(-2.0 * log(u1));
+  double sn, cs;
+  sincospi(2.0 * u2, &sn, &cs);
   g0 = r * cs;
   g1 = r * sn;
 }
@@ -320,8 +325,8 @@ __device__ __forceinline__ void rng_draw(unsigned long long seed, long long step
          (uint32_t)((unsigned long long)step >> 32)};
   const U4 r = philox(ctr, (uint32_t)seed, (uint32_t)(seed >> 32));
   T g0, g1, g2, g3;
-  box_muller<T>(r.x, r.y, g0, g1);
-  box_muller<T>(r.z, r.w, g2, g3);
+  box_muller(r.x, r.y, g0, g1);
+  box_muller(r.z, r.w, g2, g3);
   w1 = mk<T>(g0, g1);
   w2 = mk<T>(g2, g3);
 }
@@ -362,6 +367,9 @@ struct StageArgs {
   int integ, stage, last_stage, save_u0;
   double ca, cb, dt, dt_sqrt;
   int noise_on, noise_type, use_rng;
+  // on-device draws feed one stage of the step only: w1 + a w2 is drawn as
+  // sqrt(1 + a^2) w, one complex normal per entry instead of two
+  int single_draw;
   double noise_a, noise_mag, noise_norm;
   int mask_nyq;
   double fk2min, fk2max;
@@ -529,6 +537,18 @@ __device__ __forceinline__ void stage_point(const StageArgs& a, long long idx, i
     const cplx<T>* raw = a.noise ? (const cplx<T>*)a.noise + (long long)b * 2 * nc * a.nk + idx : nullptr;
     cplx<T> w[D * D];
     const long long stp = a.msteps ? a.msteps[b] : a.step;
+    // entry e of the on-device stream: w1 + a w2, or sqrt(1 + a^2) w with
+    // one draw when the step has a single noise stage
+    const T sfac = fac * (T)sqrt(1.0 + a.noise_a * a.noise_a);
+    auto draw = [&](int e) -> cplx<T> {
+      cplx<T> w1, w2;
+      if (a.single_draw) {
+        rng_draw<T>(seed, stp, idx, e >> 1, w1, w2);
+        return scl<T>((e & 1) ? w2 : w1, sfac);
+      }
+      rng_draw<T>(seed, stp, idx, e, w1, w2);
+      return add<T>(scl<T>(w1, fac), scl<T>(scl<T>(w2, fac), na));
+    };
     if (!raw && a.noise_type == 0) {
       // On-device stream, thermal noise: draw the independent entries of the
       // symmetric tensor directly.  W_ij + W_ji over sqrt 2 of two independent
@@ -539,9 +559,7 @@ __device__ __forceinline__ void stage_point(const StageArgs& a, long long idx, i
       for (int i = 0; i < D; ++i) {
 #pragma unroll
         for (int j = i; j < D; ++j) {
-          cplx<T> w1, w2;
-          rng_draw<T>(seed, stp, idx, c++, w1, w2);
-          const cplx<T> g = add<T>(scl<T>(w1, fac), scl<T>(scl<T>(w2, fac), na));
+          const cplx<T> g = draw(c++);
           // stored so that the symmetrisation below reproduces g off the
           // diagonal and sqrt(2) g on it
           const T h = (T)0.70710678118654752440;
@@ -549,13 +567,14 @@ __device__ __forceinline__ void stage_point(const StageArgs& a, long long idx, i
           w[j * D + i] = w[i * D + j];
         }
       }
+    } else if (!raw) {
+#pragma unroll
+      for (int c = 0; c < D; ++c) w[c] = draw(c);
     } else {
 #pragma unroll
       for (int c = 0; c < D * D; ++c) {
         if (c < nc) {
-          cplx<T> w1, w2;
-          if (raw) { w1 = raw[(long long)c * a.nk]; w2 = raw[(long long)(nc + c) * a.nk]; }
-          else rng_draw<T>(seed, stp, idx, c, w1, w2);
+          const cplx<T> w1 = raw[(long long)c * a.nk], w2 = raw[(long long)(nc + c) * a.nk];
           w[c] = add<T>(scl<T>(w1, fac), scl<T>(scl<T>(w2, fac), na));
         }
       }

@@ -184,6 +184,25 @@ def test_rng_noise_statistics(pf):
     assert torch.equal(out, out2)
 
 
+def test_rng_fp32_draws_match_fp64(pf):
+    """The same Philox integers feed both precisions: the FP32 Box-Muller map
+    (hardware log2, sin and cos) agrees with the FP64 library map to FP32
+    rounding, and FP32 keeps unit variance."""
+    N = (64, 64)
+    kw = dict(nonlinear=False, batch=2, noise_mag=1.0, seeds=[7, 8])
+    x = {}
+    for prec in ("fp32", "fp64"):
+        plan = pf.Plan(N, 0.01, precision=prec, **kw)
+        out = torch.empty(plan.noise_shape(), dtype=plan.complex_dtype, device="cuda")
+        plan.draw_noise(out)
+        x[prec] = _host(out).astype(np.complex128)
+    err = np.abs(x["fp32"] - x["fp64"])
+    _say("fp32 vs fp64 draws: max abs err", err.max())
+    assert err.max() < 2e-5
+    re = x["fp32"].real.ravel()
+    assert abs(re.var() - 1) < 0.03
+
+
 def test_forcing_parity_fp64(pf):
     N = (8, 8, 8)
     _say("forcing")
