@@ -202,7 +202,7 @@ template <typename T>
 int col_plain(const pf_plan* p, int L, int kind, ColArgs a, int cols, int n_outer, long long in_outer,
               long long out_outer, hipStream_t s) {
   ColGeom& g = a.g[0];
-  g.n2c = g.cvalid = g.cols = cols;
+  g.n2c = g.cload = g.cvalid = g.cols = cols;
   g.n_outer = n_outer;
   g.in_outer = in_outer;
   g.out_outer = out_outer;
@@ -215,15 +215,22 @@ template <typename T> int tile_width(int L, int kind) {
   return (kind == K_FWD_STAGE3 || kind == K_FWD_STAGE2) ? col_stage_width<T>(L) : col_tc<T>(L);
 }
 
+// Loaded / stored column counts of a geometry: the first `nvalid` entries,
+// widened to whole tiles where a row holds at least `cap` entries (cap = 0:
+// not widened).  Entries past nvalid then read whatever the padded row holds
+// and their results land in padding that no later pass reads.
+inline int widen(int nvalid, int n2c, int cap) { return cap > 0 ? std::max(nvalid, std::min(n2c, cap)) : nvalid; }
+
 // One strided-axis pass over groups of last-axis entries: `ngroups` groups
 // (rows of the axis not transformed) of `nvalid` entries each, group strides
 // in_lp / out_lp.  Tiles cover the first `nvalid` entries rounded up to whole
-// tiles (entries past nvalid are skipped); with `nyq` > 0 the entry l = nyq of
-// every group forms a second, narrow geometry of the same launch, so every
-// main tile is whole and 128-byte aligned.
+// tiles; loads and stores cover them up to the row capacities cap_in /
+// cap_out (see widen).  With `nyq` > 0 the entry l = nyq of every group forms
+// a second, narrow geometry of the same launch, so every main tile is whole
+// and 128-byte aligned.
 template <typename T>
 int col_groups(pf_plan* p, int L, int kind, ColArgs a, int ngroups, long long in_lp, long long out_lp, int nvalid,
-               int nyq, hipStream_t s) {
+               int nyq, hipStream_t s, int cap_in = 0, int cap_out = 0) {
   const int tc = tile_width<T>(L, kind);
   if (tc <= 0) return -4;
   // the fused stage kinds address the state by one flat column index
@@ -231,7 +238,8 @@ int col_groups(pf_plan* p, int L, int kind, ColArgs a, int ngroups, long long in
     return fail(PF_EINVAL, "stage pass needs one flat column group");
   ColGeom& g = a.g[0];
   g.n2c = (nvalid + tc - 1) / tc * tc;
-  g.cvalid = nvalid;
+  g.cload = widen(nvalid, g.n2c, cap_in);
+  g.cvalid = widen(nvalid, g.n2c, cap_out);
   g.cols = ngroups * g.n2c;
   g.l0 = 0;
   g.in_lp = in_lp;
@@ -243,7 +251,7 @@ int col_groups(pf_plan* p, int L, int kind, ColArgs a, int ngroups, long long in
   if (nyq > 0) {
     ColGeom& n = a.g[1];
     n = g;
-    n.n2c = n.cvalid = 1;
+    n.n2c = n.cload = n.cvalid = 1;
     n.cols = ngroups;
     n.l0 = nyq;
     n.in_off = n.out_off = nyq;
@@ -257,12 +265,13 @@ int col_groups(pf_plan* p, int L, int kind, ColArgs a, int ngroups, long long in
 // planes as its groups.
 template <typename T>
 int col_planes(pf_plan* p, int L, int kind, ColArgs a, int n_outer, long long in_outer, long long out_outer,
-               int nvalid, int nyq, hipStream_t s) {
+               int nvalid, int nyq, hipStream_t s, int cap_in = 0, int cap_out = 0) {
   const int tc = tile_width<T>(L, kind);
   if (tc <= 0) return -4;
   ColGeom& g = a.g[0];
   g.n2c = (nvalid + tc - 1) / tc * tc;
-  g.cvalid = nvalid;
+  g.cload = widen(nvalid, g.n2c, cap_in);
+  g.cvalid = widen(nvalid, g.n2c, cap_out);
   g.cols = g.n2c;
   g.l0 = 0;
   g.in_lp = g.out_lp = 0;
@@ -274,7 +283,7 @@ int col_planes(pf_plan* p, int L, int kind, ColArgs a, int n_outer, long long in
   if (nyq > 0) {
     ColGeom& n = a.g[1];
     n = g;
-    n.n2c = n.cvalid = 1;
+    n.n2c = n.cload = n.cvalid = 1;
     n.cols = n_outer;
     n.in_lp = in_outer;
     n.out_lp = out_outer;
@@ -321,10 +330,13 @@ template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStrea
     }
     {
       PassTimer t(p, PF_PASS_INV0, s);
+      // 2/3 rule: the entries past N2/3 load as zero (the mask) and are
+      // stored as zeros up to whole tiles of A's padded rows
+      const int cap = pr ? (int)P : 0;
       a.f_base = 0;
-      PF_TRY(col_groups<T>(p, M0, K_INV_U3, a, ngroups, nh, P, lval, nyq, s));
+      PF_TRY(col_groups<T>(p, M0, K_INV_U3, a, ngroups, nh, P, lval, nyq, s, 0, cap));
       a.f_base = 3;
-      PF_TRY(col_groups<T>(p, M0, K_INV_CURL3, a, ngroups, nh, P, lval, nyq, s));
+      PF_TRY(col_groups<T>(p, M0, K_INV_CURL3, a, ngroups, nh, P, lval, nyq, s, 0, cap));
     }
     // P2: axis-1 inverse, A -> Bbuf (w2)
     a = col_args();
@@ -340,7 +352,10 @@ template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStrea
     }
     {
       PassTimer t(p, PF_PASS_INV1, s);
-      PF_TRY(col_planes<T>(p, M1, K_INV, a, M0, (long long)N1 * P, (long long)M1 * P, lval, nyq, s));
+      // 2/3 rule: whole tiles of the padded rows; B's entries past N2/3 are
+      // not read by the last-axis pass
+      const int cap = pr ? (int)P : 0;
+      PF_TRY(col_planes<T>(p, M1, K_INV, a, M0, (long long)N1 * P, (long long)M1 * P, lval, nyq, s, cap, cap));
     }
     // P3: last axis c2r, u x omega, r2c, truncate: Bbuf -> C (w1)
     RowArgs r{};
@@ -364,7 +379,8 @@ template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStrea
     a.n_fields = 3; a.n_members = B; a.scale = 1.0 / M1;
     {
       PassTimer t(p, PF_PASS_FWD1, s);
-      PF_TRY(col_planes<T>(p, M1, K_FWD, a, M0, (long long)M1 * P, (long long)N1 * nh, (int)nh, 0, s));
+      // whole tiles are loaded from C's padded rows; D's rows hold nh entries
+      PF_TRY(col_planes<T>(p, M1, K_FWD, a, M0, (long long)M1 * P, (long long)N1 * nh, (int)nh, 0, s, (int)P, 0));
     }
   } else {
     const int N0 = p->N[0], N1 = p->N[1];

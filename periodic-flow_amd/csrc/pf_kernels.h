@@ -873,11 +873,13 @@ enum ColKind {
 
 // Column enumeration of one launch geometry: column cc = jj * n2c + l of
 // the tile space sits at in_off + jj * in_lp + l in the input and
-// out_off + jj * out_lp + l in the output; columns with l >= cvalid are
-// neither loaded nor stored; l0 is the last-axis index of l = 0.
+// out_off + jj * out_lp + l in the output; columns with l < cload are
+// loaded (the rest read as zero), columns with l < cvalid are stored; l0
+// is the last-axis index of l = 0.  Loads and stores may cover whole tiles
+// past the meaningful columns where the padded rows have room for them.
 struct ColGeom {
   int cols;                     // enumerated columns per matrix
-  int n2c, cvalid, l0;
+  int n2c, cload, cvalid, l0;
   int n_outer;                  // matrices per field
   int n_tiles;                  // column tiles per matrix (set by the launcher)
   long long in_lp, out_lp;
@@ -978,8 +980,9 @@ __global__ void __launch_bounds__(kThreads) col_pass(ColArgs a, const cplx<T>* _
     else { jj = cc / gm.n2c; l = cc % gm.n2c; }
     j = (a.jcompact && jj > a.K1) ? jj + a.gN1 - (2 * a.K1 + 1) : jj;
   };
-  // LDS column cf -> field, tile column, (j, l), and whether it is stored
-  struct Slot { int f, c, j, l; bool ok; };
+  // LDS column cf -> field, tile column, (j, l), whether it is loaded (ld)
+  // and stored (ok)
+  struct Slot { int f, c, j, l; bool ld, ok; };
   auto slot = [&](int cf) {
     Slot r;
     r.f = STAGE ? cf / TC : fg;
@@ -989,12 +992,14 @@ __global__ void __launch_bounds__(kThreads) col_pass(ColArgs a, const cplx<T>* _
       // the stage kinds run over one flat group (checked by the launcher)
       r.j = 0;
       r.l = cc;
+      r.ld = cc < gm.cload;
       r.ok = cc < gm.cvalid;
       return r;
     }
-    r.ok = cc < gm.cols;
-    cpos(r.ok ? cc : 0, r.j, r.l);
-    r.ok = r.ok && r.l < gm.cvalid;
+    const bool in = cc < gm.cols;
+    cpos(in ? cc : 0, r.j, r.l);
+    r.ld = in && r.l < gm.cload;
+    r.ok = in && r.l < gm.cvalid;
     return r;
   };
   // With W dividing the block size every thread touches one LDS column.
@@ -1025,7 +1030,7 @@ __global__ void __launch_bounds__(kThreads) col_pass(ColArgs a, const cplx<T>* _
       if (base + q >= NIT || idx >= L * W) continue;
       const int e = idx / W;
       const Slot sl = HOIST ? ts : slot(idx % W);
-      if (!sl.ok) continue;
+      if (!sl.ld) continue;
       const int f = sl.f, j = sl.j;
       int l = sl.l;
       const long long col = (long long)j * gm.in_lp + l;
@@ -1113,7 +1118,7 @@ __global__ void __launch_bounds__(kThreads) col_pass(ColArgs a, const cplx<T>* _
       // modes of the tile: fetch u_hat and u0 of SCH modes, then update them.
       // State index of (row o, group j, column l): (o * N1 + j) * nh + l0 + l
       // (2D: o * nh + l0 + l).
-      constexpr int SCH = 4;
+      constexpr int SCH = 2;
       const long long rowlen = NFLD == 3 ? (long long)a.gN1 * a.gnh : (long long)a.gnh;
       // TC divides the block size: one tile column per thread
       const int c = threadIdx.x % TC;
