@@ -126,6 +126,7 @@ def batched_vs_plain(torch, members=256, steps=20, plain_members=32):
     for m in range(m3):
         ub[m].copy_(torch.as_tensor(inits[m], device="cuda", dtype=pb.complex_dtype))
     u_start = ub.clone()
+    pb.autotune(ub)
     pb.step(ub, 1)
     ub.copy_(u_start)
     rb = _rate(torch, lambda: pb.step(ub, st3), n3**3 * m3 * st3)
@@ -137,6 +138,8 @@ def batched_vs_plain(torch, members=256, steps=20, plain_members=32):
             up.copy_(u_start[m:m + 1])
             pp.step(up, st3)
 
+    up.copy_(u_start[0:1])
+    pp.autotune(up)
     pp.step(up, 1)
     rp = _rate(torch, plain3, n3**3 * 8 * st3)
     out["forced3d_64_rk3"] = {"members": m3, "steps": st3, "batched": round(rb, 1), "plain": round(rp, 1),
@@ -175,6 +178,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-check", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch plans")
     ap.add_argument("--suite", default=None, help="also run the long configurations, results to this JSON file")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 1 GiB device-to-device copy (the counter calibration kernel)")
@@ -193,6 +197,14 @@ def main():
     plan = Plan([N] * 3, DT, precision="fp32", integrator="rk4", viscosity=NU, dealias="2/3")
     u_init = _tgv_state(plan, torch)
     uh = u_init.clone()
+    tune_s = None
+    if not args.no_autotune:
+        # launch plans chosen by timing (a one-off, like FFTW_MEASURE planning)
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        plan.autotune(u_init)
+        tune_s = time.perf_counter() - ta
+    tiles, fused_plan = plan.tiles
     plan.step(uh, args.warmup)
     uh.copy_(u_init)
     torch.cuda.synchronize()
@@ -284,6 +296,8 @@ def main():
         },
         "step_roofline_frac": round(step_b / (sec / args.steps) / 1e9 / perfmodel.HBM_PEAK_GBS, 4),
         "passes": per_pass,
+        "launch_plan": {"tiles": {k: v for k, v in tiles.items() if v}, "fused_stage": fused_plan,
+                        "autotune_s": None if tune_s is None else round(tune_s, 3)},
         "fp32_rel_l2_vs_fp64": err,
         "energy_end": energy32,
     }

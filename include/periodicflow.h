@@ -185,6 +185,24 @@ int pf_profile_read(pf_plan* plan, double* total_ms, int64_t* count, int32_t n);
  * per-stage API uses).  Results agree to rounding. */
 int pf_set_fusion(pf_plan* plan, int32_t fuse_stage);
 
+/* Launch plans per pass (no counterpart in the reference, whose transforms
+ * are planned by shenfun/FFTW: fft_plan="FFTW_MEASURE", pf/simulation.py:76,
+ * pf/basis.py:63-65).  tiles[PF_NPASS], indexed by PF_PASS_*: 0 is the
+ * default tile width (column passes) or row-pair count (row passes); FP32
+ * plans also have 1 (half) and 2 (twice) where they fit in LDS.  A choice
+ * changes only how work is mapped to threads: results are bit-identical.
+ * pf_set_tiles fails with PF_EUNSUPPORTED for a variant that does not exist
+ * at this plan's lengths; pf_get_tiles returns the plan's choices and
+ * whether the stage pass is fused. */
+int pf_set_tiles(pf_plan* plan, const int32_t* tiles);
+int pf_get_tiles(const pf_plan* plan, int32_t* tiles, int32_t* fuse_stage);
+/* Times every variant of every pass of the nonlinear right-hand side and
+ * the fused against the unfused axis-0 forward pass on a copy of u_hat
+ * (left unchanged), `reps` evaluations each, and keeps the fastest; the
+ * FFTW_MEASURE counterpart.  Enqueued on `stream` and synchronised;
+ * ms_out[PF_NPASS] (nullable) receives the chosen passes' average times. */
+int pf_autotune(pf_plan* plan, const void* u_hat, int32_t reps, double* ms_out, void* stream);
+
 const char* pf_last_error(void);
 const char* pf_version(void);
 

@@ -15,13 +15,18 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "periodicflow", "libpfhip.so")
 OBJ = os.path.join(HERE, "_build")
 
+# one translation unit per kernel family, precision and launch variant
 SOURCES = [
+    "pf_col_f32_v0.hip",
+    "pf_col_f32_v1.hip",
+    "pf_col_f32_v2.hip",
+    "pf_col_f64.hip",
+    "pf_row_f32_v0.hip",
+    "pf_row_f32_v1.hip",
+    "pf_row_f32_v2.hip",
+    "pf_row_f64.hip",
     "pf_api.hip",
     "pf_point.hip",
-    "pf_col_f32.hip",
-    "pf_col_f64.hip",
-    "pf_row_f32.hip",
-    "pf_row_f64.hip",
 ]
 
 FLAGS = [

@@ -76,6 +76,7 @@ struct pf_plan {
   // per-pass timing with HIP events (pf_profile_enable)
   bool prof = false;
   bool fuse_stage = true;   // axis-0 forward pass fused with the stage update
+  int tsel[PF_NPASS] = {};  // launch variant per pass (pf_set_tiles, pf_autotune)
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[PF_NPASS];
 };
 
@@ -180,11 +181,9 @@ StageArgs geometry(const pf_plan* p) {
   return g;
 }
 
-template <typename T> int col(const pf_plan* p, int L, int kind, const ColArgs& a, hipStream_t s) {
-  return launch_col<T>(L, kind, a, p->tw.at(L), s);
-}
-template <typename T> int row(const pf_plan* p, int L, int mode, const RowArgs& a, hipStream_t s) {
-  return launch_row<T>(L, mode, a, p->tw.at(L), s);
+// v: the launch variant (0 = default; the autotuner's choice per pass)
+template <typename T> int row(const pf_plan* p, int L, int mode, int v, const RowArgs& a, hipStream_t s) {
+  return launch_row<T>(L, mode, a, p->tw.at(L), s, v);
 }
 
 ColArgs col_args() {
@@ -207,12 +206,7 @@ int col_plain(const pf_plan* p, int L, int kind, ColArgs a, int cols, int n_oute
   g.in_outer = in_outer;
   g.out_outer = out_outer;
   a.ng = 1;
-  return launch_col<T>(L, kind, a, p->tw.at(L), s);
-}
-
-// Tile width a column pass of this kind uses at length L.
-template <typename T> int tile_width(int L, int kind) {
-  return (kind == K_FWD_STAGE3 || kind == K_FWD_STAGE2) ? col_stage_width<T>(L) : col_tc<T>(L);
+  return launch_col<T>(L, kind, a, p->tw.at(L), s, 0);
 }
 
 // Loaded / stored column counts of a geometry: the first `nvalid` entries,
@@ -229,9 +223,9 @@ inline int widen(int nvalid, int n2c, int cap) { return cap > 0 ? std::max(nvali
 // a second, narrow geometry of the same launch, so every main tile is whole
 // and 128-byte aligned.
 template <typename T>
-int col_groups(pf_plan* p, int L, int kind, ColArgs a, int ngroups, long long in_lp, long long out_lp, int nvalid,
-               int nyq, hipStream_t s, int cap_in = 0, int cap_out = 0) {
-  const int tc = tile_width<T>(L, kind);
+int col_groups(pf_plan* p, int L, int kind, int v, ColArgs a, int ngroups, long long in_lp, long long out_lp,
+               int nvalid, int nyq, hipStream_t s, int cap_in = 0, int cap_out = 0) {
+  const int tc = tile_cols<T>(L, kind, v);
   if (tc <= 0) return -4;
   // the fused stage kinds address the state by one flat column index
   if ((kind == K_FWD_STAGE3 || kind == K_FWD_STAGE2) && (ngroups != 1 || nyq > 0))
@@ -257,16 +251,16 @@ int col_groups(pf_plan* p, int L, int kind, ColArgs a, int ngroups, long long in
     n.in_off = n.out_off = nyq;
     a.ng = 2;
   }
-  return launch_col<T>(L, kind, a, p->tw.at(L), s);
+  return launch_col<T>(L, kind, a, p->tw.at(L), s, v);
 }
 
 // The same over the last-axis columns of per-plane matrices (the axis-1
 // passes): `n_outer` matrices of columns l; the Nyquist geometry takes the
 // planes as its groups.
 template <typename T>
-int col_planes(pf_plan* p, int L, int kind, ColArgs a, int n_outer, long long in_outer, long long out_outer,
+int col_planes(pf_plan* p, int L, int kind, int v, ColArgs a, int n_outer, long long in_outer, long long out_outer,
                int nvalid, int nyq, hipStream_t s, int cap_in = 0, int cap_out = 0) {
-  const int tc = tile_width<T>(L, kind);
+  const int tc = tile_cols<T>(L, kind, v);
   if (tc <= 0) return -4;
   ColGeom& g = a.g[0];
   g.n2c = (nvalid + tc - 1) / tc * tc;
@@ -293,7 +287,7 @@ int col_planes(pf_plan* p, int L, int kind, ColArgs a, int n_outer, long long in
     n.in_off = n.out_off = nyq;
     a.ng = 2;
   }
-  return launch_col<T>(L, kind, a, p->tw.at(L), s);
+  return launch_col<T>(L, kind, a, p->tw.at(L), s, v);
 }
 
 // Inverse passes, the fused last-axis pass and (3D) the axis-1 forward pass
@@ -334,9 +328,9 @@ template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStrea
       // stored as zeros up to whole tiles of A's padded rows
       const int cap = pr ? (int)P : 0;
       a.f_base = 0;
-      PF_TRY(col_groups<T>(p, M0, K_INV_U3, a, ngroups, nh, P, lval, nyq, s, 0, cap));
+      PF_TRY(col_groups<T>(p, M0, K_INV_U3, p->tsel[PF_PASS_INV0], a, ngroups, nh, P, lval, nyq, s, 0, cap));
       a.f_base = 3;
-      PF_TRY(col_groups<T>(p, M0, K_INV_CURL3, a, ngroups, nh, P, lval, nyq, s, 0, cap));
+      PF_TRY(col_groups<T>(p, M0, K_INV_CURL3, p->tsel[PF_PASS_INV0], a, ngroups, nh, P, lval, nyq, s, 0, cap));
     }
     // P2: axis-1 inverse, A -> Bbuf (w2)
     a = col_args();
@@ -355,7 +349,7 @@ template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStrea
       // 2/3 rule: whole tiles of the padded rows; B's entries past N2/3 are
       // not read by the last-axis pass
       const int cap = pr ? (int)P : 0;
-      PF_TRY(col_planes<T>(p, M1, K_INV, a, M0, (long long)N1 * P, (long long)M1 * P, lval, nyq, s, cap, cap));
+      PF_TRY(col_planes<T>(p, M1, K_INV, p->tsel[PF_PASS_INV1], a, M0, (long long)N1 * P, (long long)M1 * P, lval, nyq, s, cap, cap));
     }
     // P3: last axis c2r, u x omega, r2c, truncate: Bbuf -> C (w1)
     RowArgs r{};
@@ -366,7 +360,7 @@ template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStrea
     r.in_member = 6 * B_f; r.out_member = 3 * B_f;
     r.n_members = B; r.n_fields = 1;
     r.mask23 = pr; r.scale = 1.0 / M2;
-    { PassTimer t(p, PF_PASS_ROW, s); PF_TRY(row<T>(p, M2, ROW_FUSED3, r, s)); }
+    { PassTimer t(p, PF_PASS_ROW, s); PF_TRY(row<T>(p, M2, ROW_FUSED3, p->tsel[PF_PASS_ROW], r, s)); }
     // P4: axis-1 forward + truncation: C -> D (w2), all nh entries.  D has
     // the state's layout, so the axis-0 forward reads and writes the same
     // flat column index.
@@ -380,7 +374,7 @@ template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStrea
     {
       PassTimer t(p, PF_PASS_FWD1, s);
       // whole tiles are loaded from C's padded rows; D's rows hold nh entries
-      PF_TRY(col_planes<T>(p, M1, K_FWD, a, M0, (long long)M1 * P, (long long)N1 * nh, (int)nh, 0, s, (int)P, 0));
+      PF_TRY(col_planes<T>(p, M1, K_FWD, p->tsel[PF_PASS_FWD1], a, M0, (long long)M1 * P, (long long)N1 * nh, (int)nh, 0, s, (int)P, 0));
     }
   } else {
     const int N0 = p->N[0], N1 = p->N[1];
@@ -402,10 +396,10 @@ template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStrea
       PassTimer t(p, PF_PASS_INV0, s);
       a.n_fields = 2;
       a.f_base = 0;
-      PF_TRY(col_groups<T>(p, M0, K_INV_U2, a, 1, 0, 0, lval, 0, s));
+      PF_TRY(col_groups<T>(p, M0, K_INV_U2, p->tsel[PF_PASS_INV0], a, 1, 0, 0, lval, 0, s));
       a.n_fields = 1;
       a.f_base = 2;
-      PF_TRY(col_groups<T>(p, M0, K_INV_CURL2, a, 1, 0, 0, lval, 0, s));
+      PF_TRY(col_groups<T>(p, M0, K_INV_CURL2, p->tsel[PF_PASS_INV0], a, 1, 0, 0, lval, 0, s));
     }
     // P2: fused last axis: A -> C (2 fields, w2)
     RowArgs r{};
@@ -416,7 +410,7 @@ template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStrea
     r.in_member = 3 * A_f; r.out_member = 2 * A_f;
     r.n_members = B; r.n_fields = 1;
     r.mask23 = pr; r.scale = 1.0 / M1;
-    { PassTimer t(p, PF_PASS_ROW, s); PF_TRY(row<T>(p, M1, ROW_FUSED2, r, s)); }
+    { PassTimer t(p, PF_PASS_ROW, s); PF_TRY(row<T>(p, M1, ROW_FUSED2, p->tsel[PF_PASS_ROW], r, s)); }
   }
   return 0;
 }
@@ -426,6 +420,7 @@ template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStrea
 // kinds, the stage update is applied to u_hat (out unused).
 template <typename T> int last_forward(pf_plan* p, int kind, void* out, const StageArgs* st, hipStream_t s) {
   const long long nh = p->nh, P = p->pitch;
+  const int v = p->tsel[kind == K_FWD ? PF_PASS_FWD0 : PF_PASS_FWD0_STAGE];
   const int d = p->dim;
   const int M0 = p->M[0], N0 = p->N[0];
   ColArgs a = col_args();
@@ -441,13 +436,13 @@ template <typename T> int last_forward(pf_plan* p, int kind, void* out, const St
     a.in_row = a.out_row = (long long)N1 * nh;
     a.in_field = D_f; a.in_member = d * D_f;
     a.gN1 = N1; a.gnh = (int)nh;
-    return col_groups<T>(p, M0, kind, a, 1, 0, 0, N1 * (int)nh, 0, s);
+    return col_groups<T>(p, M0, kind, v, a, 1, 0, 0, N1 * (int)nh, 0, s);
   }
   const long long D_f = (long long)M0 * P;
   a.in_row = P; a.out_row = nh;
   a.in_field = D_f; a.in_member = d * D_f;
   a.gN1 = 1; a.gnh = (int)nh;
-  return col_groups<T>(p, M0, kind, a, 1, 0, 0, (int)nh, 0, s);
+  return col_groups<T>(p, M0, kind, v, a, 1, 0, 0, (int)nh, 0, s);
 }
 
 // Nonlinear term (unprojected) of u_hat into the plan's R buffer (w1).
@@ -484,7 +479,7 @@ template <typename T> int backward_fields(pf_plan* p, const void* in, void* out,
     r.in_row = nh; r.out_row = N2;
     r.in_field = p->nk; r.out_field = p->nphys;
     r.n_fields = nf; r.n_members = 1; r.out_scale = 1.0;
-    PF_TRY(row<T>(p, N2, ROW_C2R, r, s));
+    PF_TRY(row<T>(p, N2, ROW_C2R, 0, r, s));
   } else {
     const int N0 = p->N[0], N1 = p->N[1];
     ColArgs a = col_args();
@@ -500,7 +495,7 @@ template <typename T> int backward_fields(pf_plan* p, const void* in, void* out,
     r.in_row = nh; r.out_row = N1;
     r.in_field = p->nk; r.out_field = p->nphys;
     r.n_fields = nf; r.n_members = 1; r.out_scale = 1.0;
-    PF_TRY(row<T>(p, N1, ROW_C2R, r, s));
+    PF_TRY(row<T>(p, N1, ROW_C2R, 0, r, s));
   }
   return 0;
 }
@@ -515,7 +510,7 @@ template <typename T> int forward_fields(pf_plan* p, const void* in, void* out, 
     r.in_row = N2; r.out_row = nh;
     r.in_field = p->nphys; r.out_field = p->nk;
     r.n_fields = nf; r.n_members = 1; r.scale = 1.0 / N2;
-    PF_TRY(row<T>(p, N2, ROW_R2C, r, s));
+    PF_TRY(row<T>(p, N2, ROW_R2C, 0, r, s));
     ColArgs a = col_args();
     a.in = out; a.out = out;
     a.rows_in = N1; a.rows_out = N1;
@@ -538,7 +533,7 @@ template <typename T> int forward_fields(pf_plan* p, const void* in, void* out, 
     r.in_row = N1; r.out_row = nh;
     r.in_field = p->nphys; r.out_field = p->nk;
     r.n_fields = nf; r.n_members = 1; r.scale = 1.0 / N1;
-    PF_TRY(row<T>(p, N1, ROW_R2C, r, s));
+    PF_TRY(row<T>(p, N1, ROW_R2C, 0, r, s));
     ColArgs a = col_args();
     a.in = out; a.out = out;
     a.rows_in = N0; a.rows_out = N0;
@@ -630,7 +625,8 @@ template <typename T> int step_impl(pf_plan* p, void* uhat, long long n_steps, c
   const int ns = n_stages(p->d.integrator);
   const bool noise_on = p->d.noise_mag != 0.0 && uses_noise(p->d.integrator);
   const long long noise_block = (long long)p->nact * 2 * p->nc * p->nk;  // complex per step
-  const bool fused = p->fuse_stage && col_stage_width<T>(p->M[0]) > 0;
+  const bool fused = p->fuse_stage &&
+                     tile_cols<T>(p->M[0], p->dim == 3 ? K_FWD_STAGE3 : K_FWD_STAGE2, p->tsel[PF_PASS_FWD0_STAGE]) > 0;
   // Stokes flow with implicit PC and no forcing: the predictor's result is
   // never read (the corrector uses u0, L u0 and the noise only), so the step
   // is the corrector alone.
@@ -684,6 +680,118 @@ int compute_rhs_impl(pf_plan* p, void* uhat, double na, const void* noise, void*
   a.t_nl = rnl; a.t_lin = rlin; a.t_force = rf; a.t_noise = rn;
   PF_TRY(launch_stage<T>(p->dim, a, s));
   return 0;
+}
+
+}  // namespace
+
+namespace {
+
+// Whether launch variant v exists for every launch of a pass at this plan's
+// lengths (variant 0 always does).
+template <typename T> bool variant_ok(const pf_plan* p, int pass, int v) {
+  if (v == 0) return true;
+  if (v < 0 || v >= kVariants) return false;
+  const bool d3 = p->dim == 3;
+  switch (pass) {
+    case PF_PASS_INV0:
+      return d3 ? tile_cols<T>(p->M[0], K_INV_U3, v) > 0 && tile_cols<T>(p->M[0], K_INV_CURL3, v) > 0
+                : tile_cols<T>(p->M[0], K_INV_U2, v) > 0 && tile_cols<T>(p->M[0], K_INV_CURL2, v) > 0;
+    case PF_PASS_INV1: return d3 && tile_cols<T>(p->M[1], K_INV, v) > 0;
+    case PF_PASS_ROW: return row_pair_count<T>(p->M[p->dim - 1], d3 ? ROW_FUSED3 : ROW_FUSED2, v) > 0;
+    case PF_PASS_FWD1: return d3 && tile_cols<T>(p->M[1], K_FWD, v) > 0;
+    case PF_PASS_FWD0: return tile_cols<T>(p->M[0], K_FWD, v) > 0;
+    case PF_PASS_FWD0_STAGE: return tile_cols<T>(p->M[0], d3 ? K_FWD_STAGE3 : K_FWD_STAGE2, v) > 0;
+    default: return false;
+  }
+}
+
+// Average duration (ms) of the launches recorded for pass `id`; -1 if none.
+double avg_ms(pf_plan* p, int id) {
+  double t = 0.0;
+  for (auto& e : p->ev[id]) {
+    if (hipEventSynchronize(e.second) != hipSuccess) return -1.0;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, e.first, e.second) != hipSuccess) return -1.0;
+    t += ms;
+  }
+  return p->ev[id].empty() ? -1.0 : t / (double)p->ev[id].size();
+}
+
+struct DevBuffer {
+  void* ptr = nullptr;
+  ~DevBuffer() { if (ptr) (void)hipFree(ptr); }
+};
+
+// One evaluation of every tunable pass on the state `u`: the front passes,
+// the fused axis-0 forward + stage update (if it exists at this variant),
+// then the unfused axis-0 forward and stage update.
+template <typename T> int tune_round(pf_plan* p, void* u, bool fused_ok, hipStream_t s) {
+  const int ns = n_stages(p->d.integrator);
+  const bool noise_on = p->d.noise_mag != 0.0 && uses_noise(p->d.integrator);
+  StageArgs a = stage_args(p, u, ns > 1 ? 1 : 0, nullptr, noise_on);
+  a.save_u0 = 0;
+  int rc = nonlinear_front<T>(p, u, s);
+  if (rc) return rc;
+  if (fused_ok) {
+    StageArgs f = a;
+    f.rhs_nl = nullptr;
+    PassTimer t(p, PF_PASS_FWD0_STAGE, s);
+    PF_TRY(last_forward<T>(p, p->dim == 3 ? K_FWD_STAGE3 : K_FWD_STAGE2, nullptr, &f, s));
+  }
+  { PassTimer t(p, PF_PASS_FWD0, s); PF_TRY(last_forward<T>(p, K_FWD, p->w1, nullptr, s)); }
+  { PassTimer t(p, PF_PASS_STAGE, s); PF_TRY(launch_stage<T>(p->dim, a, s)); }
+  return 0;
+}
+
+template <typename T> int autotune_impl(pf_plan* p, const void* uhat, int reps, double* ms_out, hipStream_t s) {
+  if (ms_out)
+    for (int i = 0; i < PF_NPASS; ++i) ms_out[i] = 0.0;
+  if (!p->d.nonlinear) return PF_OK;  // the step is one point-wise kernel: nothing to choose
+  if (p->d.noise_mag != 0.0 && p->d.noise_type == PF_NOISE_CORRELATED && !p->corr)
+    return fail(PF_EINVAL, "correlated noise needs pf_set_correlation first");
+  const size_t sb = (size_t)p->B * p->dim * p->nk * p->csz;
+  DevBuffer tmp;
+  PF_HIP(hipMalloc(&tmp.ptr, sb));
+  PF_HIP(hipMemcpyAsync(tmp.ptr, uhat, sb, hipMemcpyDeviceToDevice, s));
+  PF_HIP(hipMemcpyAsync(p->u0, uhat, sb, hipMemcpyDeviceToDevice, s));
+  const bool prof0 = p->prof;
+  const int tunable[] = {PF_PASS_INV0, PF_PASS_INV1, PF_PASS_ROW, PF_PASS_FWD1, PF_PASS_FWD0, PF_PASS_FWD0_STAGE};
+  double best[PF_NPASS];
+  int bestv[PF_NPASS];
+  for (int i = 0; i < PF_NPASS; ++i) { best[i] = -1.0; bestv[i] = 0; }
+  double stage_ms = -1.0;
+  int rc = 0;
+  for (int v = -1; v < kVariants && !rc; ++v) {  // v = -1: warm-up round at the defaults
+    const int vv = v < 0 ? 0 : v;
+    for (int id : tunable) p->tsel[id] = variant_ok<T>(p, id, vv) ? vv : 0;
+    const bool fused_ok = tile_cols<T>(p->M[0], p->dim == 3 ? K_FWD_STAGE3 : K_FWD_STAGE2, p->tsel[PF_PASS_FWD0_STAGE]) > 0;
+    pf_profile_enable(p, 1);
+    for (int r = 0; r < (v < 0 ? 1 : reps) && !rc; ++r) rc = tune_round<T>(p, tmp.ptr, fused_ok, s);
+    if (rc || v < 0) continue;
+    for (int id : tunable) {
+      if (!variant_ok<T>(p, id, v)) continue;
+      const double t = avg_ms(p, id);
+      if (t >= 0.0 && (best[id] < 0.0 || t < best[id])) { best[id] = t; bestv[id] = v; }
+    }
+    const double ts = avg_ms(p, PF_PASS_STAGE);
+    if (ts >= 0.0 && (stage_ms < 0.0 || ts < stage_ms)) stage_ms = ts;
+  }
+  pf_profile_enable(p, prof0);
+  const hipError_t e = hipStreamSynchronize(s);
+  if (rc) {
+    for (int id : tunable) p->tsel[id] = 0;
+    return rc;
+  }
+  if (e != hipSuccess) return fail(PF_EHIP, std::string("autotune: ") + hipGetErrorString(e));
+  for (int id : tunable) p->tsel[id] = bestv[id];
+  // fused against unfused axis-0 forward pass + stage update
+  if (best[PF_PASS_FWD0_STAGE] >= 0.0 && best[PF_PASS_FWD0] >= 0.0 && stage_ms >= 0.0)
+    p->fuse_stage = best[PF_PASS_FWD0_STAGE] <= best[PF_PASS_FWD0] + stage_ms;
+  if (ms_out) {
+    for (int id : tunable) ms_out[id] = best[id] < 0.0 ? 0.0 : best[id];
+    ms_out[PF_PASS_STAGE] = stage_ms < 0.0 ? 0.0 : stage_ms;
+  }
+  return PF_OK;
 }
 
 }  // namespace
@@ -986,6 +1094,32 @@ int pf_set_fusion(pf_plan* p, int32_t fuse_stage) {
   if (!p) return fail(PF_EINVAL, "null plan");
   p->fuse_stage = fuse_stage != 0;
   return PF_OK;
+}
+
+int pf_set_tiles(pf_plan* p, const int32_t* tiles) {
+  if (!p || !tiles) return fail(PF_EINVAL, "null argument");
+  for (int i = 0; i < PF_NPASS; ++i) {
+    const bool ok = p->fp64 ? variant_ok<double>(p, i, tiles[i]) : variant_ok<float>(p, i, tiles[i]);
+    if (!ok)
+      return fail(PF_EUNSUPPORTED, "launch variant " + std::to_string(tiles[i]) + " of pass " + std::to_string(i) +
+                                       " does not exist at this plan's lengths");
+  }
+  for (int i = 0; i < PF_NPASS; ++i) p->tsel[i] = tiles[i];
+  return PF_OK;
+}
+
+int pf_get_tiles(const pf_plan* p, int32_t* tiles, int32_t* fuse_stage) {
+  if (!p || !tiles) return fail(PF_EINVAL, "null argument");
+  for (int i = 0; i < PF_NPASS; ++i) tiles[i] = p->tsel[i];
+  if (fuse_stage) *fuse_stage = p->fuse_stage ? 1 : 0;
+  return PF_OK;
+}
+
+int pf_autotune(pf_plan* p, const void* uhat, int32_t reps, double* ms_out, void* stream) {
+  if (!p || !uhat || reps < 1) return fail(PF_EINVAL, "invalid argument");
+  if (p->nact == 0) return PF_OK;
+  hipStream_t s = (hipStream_t)stream;
+  return p->fp64 ? autotune_impl<double>(p, uhat, reps, ms_out, s) : autotune_impl<float>(p, uhat, reps, ms_out, s);
 }
 
 int pf_profile_read(pf_plan* p, double* total_ms, int64_t* count, int32_t n) {

@@ -72,10 +72,20 @@ template <typename T> __device__ __forceinline__ cplx<T> pack(cplx<T> x, cplx<T>
   return mk<T>(x.x - y.y, x.y + y.x);
 }
 
-template <typename T, int L, int MODE>
+// Row-pair variants of a row pass (the autotuner's choices): V = 0 the
+// default count, 1 half of it, 2 twice it; 0 where a variant repeats another
+// or does not fit in LDS.
+template <typename T, int MODE> constexpr int row_pairs_v(int L, int V) {
+  const int np = row_pairs<T, MODE>(L);
+  const int r = V == 0 ? np : (V == 1 ? np / 2 : np * 2);
+  if (r < 1 || (V != 0 && r == np)) return 0;
+  const long long lds = ((long long)r * row_slots<MODE>() * row_pitch(L) + L) * (long long)sizeof(cplx<T>);
+  return lds <= 160 * 1024 ? r : 0;
+}
+
+template <typename T, int L, int MODE, int NP>
 __global__ void __launch_bounds__(kThreads) row_pass(RowArgs a, const cplx<T>* __restrict__ tw) {
   constexpr int NS = row_slots<MODE>();
-  constexpr int NP = row_pairs<T, MODE>(L);
   constexpr int LP = row_pitch(L);
   __shared__ cplx<T> tws[tw_alloc(L)];
   __shared__ cplx<T> buf[NP * NS * LP];
@@ -919,6 +929,17 @@ template <typename T> constexpr int col_tc_stage(int L) {
 }
 template <typename T, int KIND> constexpr int col_width(int L) {
   return (KIND == K_FWD_STAGE3 || KIND == K_FWD_STAGE2) ? col_tc_stage<T>(L) : col_tc<T>(L);
+}
+// Tile-width variants (the autotuner's choices): V = 0 the default width,
+// 1 half of it, 2 twice it; 0 where a variant does not fit in LDS.
+template <typename T, int KIND> constexpr int col_width_v(int L, int V) {
+  const int base = col_width<T, KIND>(L);
+  if (base == 0 || V == 0) return base;
+  const int tc = V == 1 ? base / 2 : base * 2;
+  if (tc < 1 || tc > 64) return 0;
+  const int nf = KIND == K_FWD_STAGE3 ? 3 : (KIND == K_FWD_STAGE2 ? 2 : 1);
+  const long long lds = ((long long)L * nf * tc + tw_alloc(L)) * (long long)sizeof(cplx<T>);
+  return lds <= 160 * 1024 ? tc : 0;
 }
 
 // Row index of the padded spectrum -> source row and factor (3/2 rule,

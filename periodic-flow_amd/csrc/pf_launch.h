@@ -12,12 +12,71 @@ namespace pf {
 
 bool length_supported(int L);
 
+// Launch variants: V = 0 is the default tile (column passes) or row-pair
+// count (row passes); FP32 also builds V = 1 (half) and V = 2 (twice), the
+// choices the autotuner times.  One translation unit per precision and V.
+constexpr int kVariants = 3;
+template <typename T, int V> int launch_col_v(int L, int kind, const ColArgs& a, const void* tw, hipStream_t s);
+template <typename T, int V> int col_width_q(int L, int kind);
+template <typename T, int V> int launch_row_v(int L, int mode, const RowArgs& a, const void* tw, hipStream_t s);
+template <typename T, int V> int row_pairs_q(int L, int mode);
+
+template <typename T> constexpr bool has_variants() { return sizeof(T) == 4; }
+
 // Launch one column pass (FFT along a strided axis) of the given ColKind.
-template <typename T> int launch_col(int L, int kind, const ColArgs& a, const void* tw, hipStream_t s);
-// Tile width of the fused forward + stage kinds at length L (0: unavailable).
-template <typename T> int col_stage_width(int L);
+template <typename T>
+int launch_col(int L, int kind, const ColArgs& a, const void* tw, hipStream_t s, int v) {
+  if constexpr (has_variants<T>()) {
+    switch (v) {
+      case 0: return launch_col_v<T, 0>(L, kind, a, tw, s);
+      case 1: return launch_col_v<T, 1>(L, kind, a, tw, s);
+      case 2: return launch_col_v<T, 2>(L, kind, a, tw, s);
+      default: return -1;
+    }
+  } else {
+    return v == 0 ? launch_col_v<T, 0>(L, kind, a, tw, s) : -4;
+  }
+}
+// Tile width of a column pass kind at length L and variant v (0: unavailable).
+template <typename T> int tile_cols(int L, int kind, int v) {
+  if constexpr (has_variants<T>()) {
+    switch (v) {
+      case 0: return col_width_q<T, 0>(L, kind);
+      case 1: return col_width_q<T, 1>(L, kind);
+      case 2: return col_width_q<T, 2>(L, kind);
+      default: return 0;
+    }
+  } else {
+    return v == 0 ? col_width_q<T, 0>(L, kind) : 0;
+  }
+}
 // Launch one row pass (last axis) of the given RowMode.
-template <typename T> int launch_row(int L, int mode, const RowArgs& a, const void* tw, hipStream_t s);
+template <typename T>
+int launch_row(int L, int mode, const RowArgs& a, const void* tw, hipStream_t s, int v) {
+  if constexpr (has_variants<T>()) {
+    switch (v) {
+      case 0: return launch_row_v<T, 0>(L, mode, a, tw, s);
+      case 1: return launch_row_v<T, 1>(L, mode, a, tw, s);
+      case 2: return launch_row_v<T, 2>(L, mode, a, tw, s);
+      default: return -1;
+    }
+  } else {
+    return v == 0 ? launch_row_v<T, 0>(L, mode, a, tw, s) : -4;
+  }
+}
+// Row pairs per block of a row pass at length L and variant v (0: unavailable).
+template <typename T> int row_pair_count(int L, int mode, int v) {
+  if constexpr (has_variants<T>()) {
+    switch (v) {
+      case 0: return row_pairs_q<T, 0>(L, mode);
+      case 1: return row_pairs_q<T, 1>(L, mode);
+      case 2: return row_pairs_q<T, 2>(L, mode);
+      default: return 0;
+    }
+  } else {
+    return v == 0 ? row_pairs_q<T, 0>(L, mode) : 0;
+  }
+}
 
 template <typename T> int launch_stage(int dim, const StageArgs& a, hipStream_t s);
 template <typename T> int launch_diag(int dim, const DiagArgs& a, int n_members, hipStream_t s);

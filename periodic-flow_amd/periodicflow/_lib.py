@@ -84,6 +84,9 @@ _SIGS = {
     "pf_profile_enable": (C.c_int, [_P, C.c_int32]),
     "pf_profile_read": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32]),
     "pf_set_fusion": (C.c_int, [_P, C.c_int32]),
+    "pf_set_tiles": (C.c_int, [_P, C.POINTER(C.c_int32)]),
+    "pf_get_tiles": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "pf_autotune": (C.c_int, [_P, _P, C.c_int32, C.POINTER(C.c_double), _P]),
     "pf_last_error": (C.c_char_p, []),
     "pf_version": (C.c_char_p, []),
 }

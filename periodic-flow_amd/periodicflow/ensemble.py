@@ -56,8 +56,9 @@ class Ensemble:
     (batch, d, *Nf) tensor per group; ``state(i)`` is member i's view.
     """
 
-    def __init__(self, members, precision="fp32", correlation=None):
+    def __init__(self, members, precision="fp32", correlation=None, autotune=False):
         self.precision = precision
+        self._tune = autotune
         self.groups: dict = {}
         self.where = []  # member id -> (group key, slot)
         order = {}
@@ -99,13 +100,24 @@ class Ensemble:
     def points_per_step(self):
         return sum(int(np.prod(g.spec.n)) * g.plan.batch for g in self.groups.values())
 
+    def autotune(self, reps=3):
+        """Choose every group's launch plans by timing (Plan.autotune); returns
+        {group key: {pass name: chosen ms}}."""
+        self._tune = False
+        self._graph = None
+        return {k: g.plan.autotune(g.state, reps) for k, g in self.groups.items()}
+
     def step(self, n_steps=1):
+        if self._tune:
+            self.autotune()
         for g in self.groups.values():
             g.plan.step(g.state, n_steps)
 
     # ---------------------------------------------------------------- graphs
     def capture(self):
         """Capture one step of every group into a HIP graph (replayed by run)."""
+        if self._tune:
+            self.autotune()
         torch.cuda.synchronize()
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):

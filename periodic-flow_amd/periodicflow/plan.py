@@ -248,6 +248,36 @@ class Plan:
         """Fuse the axis-0 forward pass with the stage update in step() (default on)."""
         call("pf_set_fusion", self._h, int(bool(fuse_stage)))
 
+    @property
+    def tiles(self):
+        """{pass name: launch variant} and whether the stage pass is fused."""
+        n = len(_lib.PASS_NAMES)
+        t = (C.c_int32 * n)()
+        fused = C.c_int32(0)
+        call("pf_get_tiles", self._h, t, C.byref(fused))
+        return {nm: int(t[i]) for i, nm in enumerate(_lib.PASS_NAMES)}, bool(fused.value)
+
+    def set_tiles(self, tiles):
+        """Launch variant per pass (dict by pass name, unnamed passes 0): 0 the
+        default tile width or row-pair count, 1 half of it, 2 twice it.  Results
+        are bit-identical across variants."""
+        n = len(_lib.PASS_NAMES)
+        unknown = set(tiles) - set(_lib.PASS_NAMES)
+        if unknown:
+            raise ValueError(f"unknown passes {sorted(unknown)}")
+        t = (C.c_int32 * n)(*[int(tiles.get(nm, 0)) for nm in _lib.PASS_NAMES])
+        call("pf_set_tiles", self._h, t)
+
+    def autotune(self, u_hat, reps=3):
+        """Time every launch variant of every nonlinear pass (and the fused
+        against the unfused stage pass) on a copy of u_hat, keep the fastest;
+        returns {pass name: chosen average ms}."""
+        self._check(u_hat, self.state_shape(), self.complex_dtype, "u_hat")
+        n = len(_lib.PASS_NAMES)
+        ms = (C.c_double * n)()
+        call("pf_autotune", self._h, _ptr(u_hat), int(reps), ms, _stream())
+        return {nm: float(ms[i]) for i, nm in enumerate(_lib.PASS_NAMES)}
+
     def profile(self, enable=True):
         """Start (or stop) per-pass HIP-event timing; clears earlier records."""
         call("pf_profile_enable", self._h, int(bool(enable)))

@@ -77,6 +77,12 @@ class Solver:
         self.time_integrator = cls(dt, self.space_solver.linear_operator, optimization)
         self.time_integrator.bind(self.space_solver.plan)
         self._wall = 0.0
+        # fft_plan (simulation.py:76) picks the planning effort: FFTW_ESTIMATE
+        # keeps the default launch plans, the measuring modes time every
+        # variant on the first advance (Plan.autotune)
+        self.fft_plan = fft_plan
+        self._tuned = str(fft_plan).upper() == "FFTW_ESTIMATE"
+        self.tuning = None
 
     def __getattr__(self, name):
         # simulation.py:116-120
@@ -127,6 +133,9 @@ class Solver:
         dt = self.time_integrator.dt
         if n_steps <= 0:
             return
+        if not self._tuned:
+            self.tuning = ss.plan.autotune(ss._u_hat)
+            self._tuned = True
         if self._fused():
             uses_noise = ss.noise_mag != 0 and any(f != 0 for f in self.time_integrator.noise_factor)
             if ss.noise_source == "numpy" and uses_noise:

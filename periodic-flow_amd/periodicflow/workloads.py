@@ -151,9 +151,11 @@ def forced3d(steps=1000, members=None, graph=False):
     ens = Ensemble(members, precision="fp32", correlation=forced_correlation)
     for i, (spec, seed) in enumerate(members):
         ens.set_fourier(i, random_solenoidal(ens.plan_of(i), seed, energy=0.5, kp=4.0))
+    tune = ens.autotune()  # tile plans chosen by timing, before the timed run
     t = _timed(lambda: ens.run(steps, graph=graph))
     return {"members": len(members), "steps": steps, "seconds": t,
-            "Mpoint_steps_per_s": ens.points_per_step() * steps / t / 1e6, "ensemble": ens}
+            "Mpoint_steps_per_s": ens.points_per_step() * steps / t / 1e6, "ensemble": ens,
+            "tiles": {str(k): g.plan.tiles for k, g in ens.groups.items()}, "tuning_ms": tune}
 
 
 def serving_requests(count=400, rate=20.0, seed=42, max_steps=2000, sizes=None):

@@ -90,6 +90,9 @@ def test_null_arguments(lib):
     assert lib.pf_plan_create(None, None) == -1
     assert lib.pf_step(None, None, 1, None, None) == -1
     assert lib.pf_get_step_counter(None) == -1
+    assert lib.pf_set_tiles(None, None) == -1
+    assert lib.pf_get_tiles(None, None, None) == -1
+    assert lib.pf_autotune(None, None, 1, None, None) == -1
     assert lib.pf_plan_destroy(None) == 0
 
 

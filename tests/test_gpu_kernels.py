@@ -236,3 +236,67 @@ def test_fused_stage_equals_unfused(pf, N, dealias, integ):
         plan.step(ud, 3)
         out.append(_host(ud))
     assert po.rel_l2(out[0], out[1]) < 1e-14
+
+
+TUNABLE = ("inv_axis0_curl", "inv_axis1", "row_c2r_cross_r2c", "fwd_axis1", "fwd_axis0", "fwd_axis0_stage")
+
+
+@pytest.mark.parametrize("N,dealias,integ", [((24, 24, 24), "3/2", "rk3"), ((32, 32, 32), "2/3", "rk4"),
+                                             ((16, 24, 32), "3/2", "rk2"), ((64, 64), "3/2", "rk3"),
+                                             ((48, 64), "2/3", "rk4")])
+def test_launch_variants_bit_identical(pf, N, dealias, integ):
+    """Every launch variant of every pass (tile width / rows per block) maps
+    the same arithmetic to other threads: the right-hand side and the fused
+    step agree bit for bit with the default plan; autotune leaves u_hat as it
+    was and picks only existing variants."""
+    _say("variants", N, dealias, integ)
+    rs = np.random.RandomState(11)
+    plan = pf.Plan(N, 1e-3, precision="fp32", integrator=integ, viscosity=0.01, dealias=dealias, batch=2,
+                   noise_mag=0.05, seeds=[3, 4])
+    u = _dev(0.1 * _rand_spec(rs, (2, len(N)) + plan.shape_fourier), plan.complex_dtype)
+    plan.project(u)
+    ref_rhs = torch.empty_like(u)
+    plan.rhs_nonlinear(u, ref_rhs)
+    ref_step = u.clone()
+    plan.step_counter = 0
+    plan.step(ref_step, 2)
+    tried = 0
+    for name in TUNABLE:
+        for v in (1, 2):
+            try:
+                plan.set_tiles({name: v})
+            except NotImplementedError:
+                continue
+            tried += 1
+            rhs = torch.empty_like(u)
+            plan.rhs_nonlinear(u, rhs)
+            assert torch.equal(rhs, ref_rhs), (name, v)
+            w = u.clone()
+            plan.step_counter = 0
+            plan.step(w, 2)
+            assert torch.equal(w, ref_step), (name, v)
+    assert tried > 0
+    plan.set_tiles({})
+    before = u.clone()
+    ms = plan.autotune(u, reps=2)
+    assert torch.equal(u, before)
+    tiles, fused = plan.tiles
+    _say("chosen", {k: v for k, v in tiles.items() if v}, "fused", fused, {k: round(x, 4) for k, x in ms.items() if x})
+    w = u.clone()
+    plan.step_counter = 0
+    plan.step(w, 2)
+    if fused:
+        assert torch.equal(w, ref_step)
+    else:
+        # the unfused sequence agrees to rounding
+        err = float(torch.linalg.vector_norm(w - ref_step) / torch.linalg.vector_norm(ref_step))
+        assert err < FP32_TOL, err
+
+
+def test_set_tiles_rejects_missing_variants(pf):
+    plan = pf.Plan((8, 8, 8), 1e-3, precision="fp64", integrator="rk3")
+    with pytest.raises(NotImplementedError):
+        plan.set_tiles({"inv_axis1": 1})
+    with pytest.raises(ValueError):
+        plan.set_tiles({"no_such_pass": 1})
+    plan.set_tiles({"inv_axis1": 0})
