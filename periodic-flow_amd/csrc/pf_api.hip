@@ -722,22 +722,25 @@ struct DevBuffer {
   ~DevBuffer() { if (ptr) (void)hipFree(ptr); }
 };
 
-// One evaluation of every tunable pass on the state `u`: the front passes,
-// the fused axis-0 forward + stage update (if it exists at this variant),
-// then the unfused axis-0 forward and stage update.
+// One evaluation of every tunable pass on the state `u`: the front passes
+// and the fused axis-0 forward + stage update (if it exists at this
+// variant), then the front passes again and the unfused axis-0 forward and
+// stage update, so that each candidate follows the same passes (whose output
+// may still sit in the last-level cache) as it does in a step.
 template <typename T> int tune_round(pf_plan* p, void* u, bool fused_ok, hipStream_t s) {
   const int ns = n_stages(p->d.integrator);
   const bool noise_on = p->d.noise_mag != 0.0 && uses_noise(p->d.integrator);
   StageArgs a = stage_args(p, u, ns > 1 ? 1 : 0, nullptr, noise_on);
   a.save_u0 = 0;
-  int rc = nonlinear_front<T>(p, u, s);
-  if (rc) return rc;
+  int rc = 0;
   if (fused_ok) {
+    if ((rc = nonlinear_front<T>(p, u, s))) return rc;
     StageArgs f = a;
     f.rhs_nl = nullptr;
     PassTimer t(p, PF_PASS_FWD0_STAGE, s);
     PF_TRY(last_forward<T>(p, p->dim == 3 ? K_FWD_STAGE3 : K_FWD_STAGE2, nullptr, &f, s));
   }
+  if ((rc = nonlinear_front<T>(p, u, s))) return rc;
   { PassTimer t(p, PF_PASS_FWD0, s); PF_TRY(last_forward<T>(p, K_FWD, p->w1, nullptr, s)); }
   { PassTimer t(p, PF_PASS_STAGE, s); PF_TRY(launch_stage<T>(p->dim, a, s)); }
   return 0;
@@ -756,9 +759,9 @@ template <typename T> int autotune_impl(pf_plan* p, const void* uhat, int reps, 
   PF_HIP(hipMemcpyAsync(p->u0, uhat, sb, hipMemcpyDeviceToDevice, s));
   const bool prof0 = p->prof;
   const int tunable[] = {PF_PASS_INV0, PF_PASS_INV1, PF_PASS_ROW, PF_PASS_FWD1, PF_PASS_FWD0, PF_PASS_FWD0_STAGE};
-  double best[PF_NPASS];
+  double best[PF_NPASS], def[PF_NPASS];
   int bestv[PF_NPASS];
-  for (int i = 0; i < PF_NPASS; ++i) { best[i] = -1.0; bestv[i] = 0; }
+  for (int i = 0; i < PF_NPASS; ++i) { best[i] = def[i] = -1.0; bestv[i] = 0; }
   double stage_ms = -1.0;
   int rc = 0;
   for (int v = -1; v < kVariants && !rc; ++v) {  // v = -1: warm-up round at the defaults
@@ -771,6 +774,7 @@ template <typename T> int autotune_impl(pf_plan* p, const void* uhat, int reps, 
     for (int id : tunable) {
       if (!variant_ok<T>(p, id, v)) continue;
       const double t = avg_ms(p, id);
+      if (v == 0) def[id] = t;
       if (t >= 0.0 && (best[id] < 0.0 || t < best[id])) { best[id] = t; bestv[id] = v; }
     }
     const double ts = avg_ms(p, PF_PASS_STAGE);
@@ -783,10 +787,20 @@ template <typename T> int autotune_impl(pf_plan* p, const void* uhat, int reps, 
     return rc;
   }
   if (e != hipSuccess) return fail(PF_EHIP, std::string("autotune: ") + hipGetErrorString(e));
-  for (int id : tunable) p->tsel[id] = bestv[id];
-  // fused against unfused axis-0 forward pass + stage update
+  // A variant replaces the default only when it is clearly faster (timing
+  // noise of a few repetitions must not flip a plan).
+  constexpr double kMargin = 0.97;
+  for (int id : tunable) {
+    if (bestv[id] != 0 && def[id] >= 0.0 && best[id] > kMargin * def[id]) {
+      bestv[id] = 0;
+      best[id] = def[id];
+    }
+    p->tsel[id] = bestv[id];
+  }
+  // fused against unfused axis-0 forward pass + stage update (fused unless
+  // the unfused pair is clearly faster)
   if (best[PF_PASS_FWD0_STAGE] >= 0.0 && best[PF_PASS_FWD0] >= 0.0 && stage_ms >= 0.0)
-    p->fuse_stage = best[PF_PASS_FWD0_STAGE] <= best[PF_PASS_FWD0] + stage_ms;
+    p->fuse_stage = best[PF_PASS_FWD0] + stage_ms > kMargin * best[PF_PASS_FWD0_STAGE];
   if (ms_out) {
     for (int id : tunable) ms_out[id] = best[id] < 0.0 ? 0.0 : best[id];
     ms_out[PF_PASS_STAGE] = stage_ms < 0.0 ? 0.0 : stage_ms;

@@ -930,16 +930,34 @@ template <typename T> constexpr int col_tc_stage(int L) {
 template <typename T, int KIND> constexpr int col_width(int L) {
   return (KIND == K_FWD_STAGE3 || KIND == K_FWD_STAGE2) ? col_tc_stage<T>(L) : col_tc<T>(L);
 }
-// Tile-width variants (the autotuner's choices): V = 0 the default width,
-// 1 half of it, 2 twice it; 0 where a variant does not fit in LDS.
+// Launch variants (the autotuner's choices): V = 0 the default tile width,
+// 1 half of it, 2 twice it, 3 the default width with two tiles per block,
+// 4 (stage kinds) twice the width with 1024-thread blocks; width 0 where a
+// variant does not fit in LDS or does not exist.
 template <typename T, int KIND> constexpr int col_width_v(int L, int V) {
   const int base = col_width<T, KIND>(L);
   if (base == 0 || V == 0) return base;
+  if (V == 3) return (KIND == K_FWD_STAGE3 || KIND == K_FWD_STAGE2) ? 0 : base;
+  if (V == 4 && !(KIND == K_FWD_STAGE3 || KIND == K_FWD_STAGE2)) return 0;
   const int tc = V == 1 ? base / 2 : base * 2;
   if (tc < 1 || tc > 64) return 0;
   const int nf = KIND == K_FWD_STAGE3 ? 3 : (KIND == K_FWD_STAGE2 ? 2 : 1);
   const long long lds = ((long long)L * nf * tc + tw_alloc(L)) * (long long)sizeof(cplx<T>);
   return lds <= 160 * 1024 ? tc : 0;
+}
+// Threads per block: 1024 for variant 4 of the fused stage kinds (twice the
+// default tile width, one block per CU holding the whole LDS), else 256.
+template <typename T, int KIND> constexpr int col_threads_v(int L, int V) {
+  return (V == 4 && (KIND == K_FWD_STAGE3 || KIND == K_FWD_STAGE2)) ? 1024 : kThreads;
+}
+// Tiles per block: 2 for variant 3 of the plain and velocity kinds, whose
+// next tile's loads (at most 32 per thread, held in registers) are issued
+// before the current tile's transform and stores; otherwise 1.
+template <typename T, int KIND> constexpr int col_pipe_v(int L, int V) {
+  if (V != 3) return 1;
+  const bool kind_ok = KIND == K_INV || KIND == K_FWD || KIND == K_INV_U3 || KIND == K_INV_U2;
+  const int tc = col_width<T, KIND>(L);
+  return kind_ok && tc > 0 && L * tc / kThreads <= 32 ? 2 : 1;
 }
 
 // Row index of the padded spectrum -> source row and factor (3/2 rule,
@@ -961,8 +979,8 @@ __device__ __forceinline__ long long xcd_order(long long bid, long long G) {
   return x * q + (x < r ? x : r) + bid / 8;
 }
 
-template <typename T, int L, int TC, int KIND>
-__global__ void __launch_bounds__(kThreads) col_pass(ColArgs a, const cplx<T>* __restrict__ tw) {
+template <typename T, int L, int TC, int KIND, int PIPE, int NT>
+__global__ void __launch_bounds__(NT) col_pass(ColArgs a, const cplx<T>* __restrict__ tw) {
   constexpr bool STAGE = KIND == K_FWD_STAGE3 || KIND == K_FWD_STAGE2;
   constexpr bool INV = KIND == K_INV || KIND == K_INV_CURL3 || KIND == K_INV_CURL2 || KIND == K_INV_U3 ||
                         KIND == K_INV_U2;
@@ -972,25 +990,29 @@ __global__ void __launch_bounds__(kThreads) col_pass(ColArgs a, const cplx<T>* _
   constexpr int GD = (KIND == K_INV_CURL3 || KIND == K_INV_U3) ? 3 : 2;
   constexpr int NFLD = KIND == K_FWD_STAGE3 ? 3 : (KIND == K_FWD_STAGE2 ? 2 : 1);
   constexpr int W = NFLD * TC;  // LDS columns
-  constexpr int NIT = (L * W + kThreads - 1) / kThreads;
+  constexpr int NIT = (L * W + NT - 1) / NT;
   constexpr int CH = NIT < 16 ? NIT : 16;  // loads kept in flight per thread
-  static_assert(kThreads % TC == 0, "tile width must divide the block size");
+  static_assert(NT % TC == 0, "tile width must divide the block size");
   __shared__ cplx<T> tws[tw_alloc(L)];
   __shared__ cplx<T> buf[L * W];
-  for (int m = threadIdx.x; m < tw_size(L); m += kThreads) tws[m] = tw[m];
+  for (int m = threadIdx.x; m < tw_size(L); m += NT) tws[m] = tw[m];
 
   const int nfg = STAGE ? 1 : a.n_fields;
   long long lg = xcd_order(blockIdx.x, gridDim.x);
   const bool nyqg = a.ng > 1 && lg < a.g[1].n_blocks;
   const ColGeom gm = nyqg ? a.g[1] : a.g[0];
   if (!nyqg && a.ng > 1) lg -= a.g[1].n_blocks;
+  // PIPE consecutive tiles of one (field, matrix, member) per block; the
+  // tile index varies fastest, so the blocks an XCD runs together cover
+  // neighbouring columns of the same rows
+  const int ntb = (gm.n_tiles + PIPE - 1) / PIPE;
+  const int tb = (int)(lg % ntb);
+  lg /= ntb;
   const int fg = (int)(lg % nfg);
   lg /= nfg;
-  const int tile = (int)(lg % gm.n_tiles);
-  lg /= gm.n_tiles;
   const int outer = (int)(lg % gm.n_outer);
   const int b = (int)(lg / gm.n_outer);
-  const int c0 = tile * TC;
+  int c0 = tb * PIPE * TC;  // first column of the tile in hand
   const cplx<T>* in = (const cplx<T>*)a.in + gm.in_off + (long long)b * a.in_member + (long long)outer * gm.in_outer;
 
   // enumerated column -> (group j, column l within the group)
@@ -1024,7 +1046,7 @@ __global__ void __launch_bounds__(kThreads) col_pass(ColArgs a, const cplx<T>* _
     return r;
   };
   // With W dividing the block size every thread touches one LDS column.
-  constexpr bool HOIST = kThreads % W == 0;
+  constexpr bool HOIST = NT % W == 0;
   Slot ts{};
   if constexpr (HOIST) ts = slot(threadIdx.x % W);
 
@@ -1041,11 +1063,13 @@ __global__ void __launch_bounds__(kThreads) col_pass(ColArgs a, const cplx<T>* _
     else if (f == 0) { ia = 2; ib = 1; ka = k1; kb = k2; }
     else { ia = 0; ib = 2; ka = k2; kb = k0; }
   };
-  for (int base = 0; base < NIT; base += CH) {
-    cplx<T> ra[CH], rb[CH];
+  // Loads of this thread's LDS slots base .. base + NQ - 1 of the tile at c0
+  // (NQ = the register arrays' length), all issued before any is used.
+  auto issue = [&](int base, auto& ra, auto& rb) {
+    constexpr int NQ = (int)(sizeof(ra) / sizeof(ra[0]));
 #pragma unroll
-    for (int q = 0; q < CH; ++q) {
-      const int idx = threadIdx.x + (base + q) * kThreads;
+    for (int q = 0; q < NQ; ++q) {
+      const int idx = threadIdx.x + (base + q) * NT;
       ra[q] = mk<T>(0, 0);
       rb[q] = mk<T>(0, 0);
       if (base + q >= NIT || idx >= L * W) continue;
@@ -1079,9 +1103,13 @@ __global__ void __launch_bounds__(kThreads) col_pass(ColArgs a, const cplx<T>* _
         ra[q] = in[(long long)f * a.in_field + (long long)src * a.in_row + col];
       }
     }
+  };
+  // ... combined (curl, 3/2-rule Nyquist halving) and stored to LDS
+  auto commit = [&](int base, const auto& ra, const auto& rb) {
+    constexpr int NQ = (int)(sizeof(ra) / sizeof(ra[0]));
 #pragma unroll
-    for (int q = 0; q < CH; ++q) {
-      const int idx = threadIdx.x + (base + q) * kThreads;
+    for (int q = 0; q < NQ; ++q) {
+      const int idx = threadIdx.x + (base + q) * NT;
       if (base + q >= NIT || idx >= L * W) continue;
       cplx<T> v = ra[q];
       if constexpr (CURL) {
@@ -1104,24 +1132,43 @@ __global__ void __launch_bounds__(kThreads) col_pass(ColArgs a, const cplx<T>* _
       }
       buf[idx] = v;  // idx = e * W + cf
     }
+  };
+  for (int base = 0; base < NIT; base += CH) {
+    cplx<T> ra[CH], rb[CH];
+    issue(base, ra, rb);
+    commit(base, ra, rb);
   }
   __syncthreads();
 
-  auto addr = [](int ff, int e) { return e * W + ff; };
-  if constexpr (INV) lds_fft<T, L, +1, W, kThreads, true>(buf, tws, addr);
-  else lds_fft<T, L, -1, W, kThreads, true>(buf, tws, addr);
-
   const T s = (T)a.scale;
+  auto addr = [](int ff, int e) { return e * W + ff; };
+  for (int it = 0; it < PIPE; ++it) {
+  // two-tile blocks: the next tile's loads fly during this tile's transform
+  // and stores (the condition is uniform over the block)
+  const bool nxt = PIPE > 1 && it + 1 < PIPE && tb * PIPE + it + 1 < gm.n_tiles;
+  cplx<T> pa[PIPE > 1 ? NIT : 1], pb[PIPE > 1 ? NIT : 1];
+  if constexpr (PIPE > 1) {
+    if (nxt) {
+      c0 += TC;
+      if constexpr (HOIST) ts = slot(threadIdx.x % W);
+      issue(0, pa, pb);
+      c0 -= TC;
+      if constexpr (HOIST) ts = slot(threadIdx.x % W);
+    }
+  }
+  if constexpr (INV) lds_fft<T, L, +1, W, NT, true>(buf, tws, addr);
+  else lds_fft<T, L, -1, W, NT, true>(buf, tws, addr);
+
   if constexpr (INV) {
     cplx<T>* out = (cplx<T>*)a.out + gm.out_off + (long long)b * a.out_member + (long long)(a.f_base + fg) * a.out_field +
                    (long long)outer * gm.out_outer;
-    constexpr int NO = (L * TC + kThreads - 1) / kThreads;
+    constexpr int NO = (L * TC + NT - 1) / NT;
     // TC divides the block size: one tile column per thread
     const Slot os = slot(threadIdx.x % TC);
     const long long ocol = (long long)os.j * gm.out_lp + os.l;
 #pragma unroll
     for (int q = 0; q < NO; ++q) {
-      const int idx = threadIdx.x + q * kThreads;
+      const int idx = threadIdx.x + q * NT;
       if (idx >= L * TC || !os.ok) continue;
       const int e = idx / TC;
       out[(long long)e * a.out_row + ocol] = scl<T>(buf[e * W + os.c], s);
@@ -1148,16 +1195,16 @@ __global__ void __launch_bounds__(kThreads) col_pass(ColArgs a, const cplx<T>* _
       // grid indices (j, l) of the column: one division per thread
       const int jt = NFLD == 3 ? (int)((unsigned)mcol / (unsigned)a.gnh) : 0;
       const int lt = (int)mcol - jt * a.gnh;
-      for (int idx0 = threadIdx.x; idx0 < n * TC; idx0 += SCH * kThreads) {
+      for (int idx0 = threadIdx.x; idx0 < n * TC; idx0 += SCH * NT) {
         cplx<T> uu[SCH][NFLD], zz[SCH][NFLD];
 #pragma unroll
         for (int q = 0; q < SCH; ++q) {
-          const int idx = idx0 + q * kThreads;
+          const int idx = idx0 + q * NT;
           if (idx < n * TC && store) stage_fetch<T, NFLD>(a.st, (long long)(idx / TC) * rowlen + mcol, b, uu[q], zz[q]);
         }
 #pragma unroll
         for (int q = 0; q < SCH; ++q) {
-          const int idx = idx0 + q * kThreads;
+          const int idx = idx0 + q * NT;
           if (idx >= n * TC || !store) continue;
           const int o = idx / TC;
           cplx<T> nl[NFLD];
@@ -1172,12 +1219,23 @@ __global__ void __launch_bounds__(kThreads) col_pass(ColArgs a, const cplx<T>* _
                      (long long)outer * gm.out_outer;
       const Slot os = slot(threadIdx.x % TC);
       const long long ocol = (long long)os.j * gm.out_lp + os.l;
-      for (int idx = threadIdx.x; idx < n * TC; idx += kThreads) {
+      for (int idx = threadIdx.x; idx < n * TC; idx += NT) {
         if (!os.ok) break;
         const int o = idx / TC;
         out[(long long)o * a.out_row + ocol] = trunc(o, os.c);
       }
     }
+  }
+  if (!nxt) break;
+  if constexpr (PIPE > 1) {
+    {
+      __syncthreads();  // every read of this tile's LDS is done
+      c0 += TC;
+      if constexpr (HOIST) ts = slot(threadIdx.x % W);
+      commit(0, pa, pb);
+      __syncthreads();
+    }
+  }
   }
 }
 

@@ -100,7 +100,7 @@ class Ensemble:
     def points_per_step(self):
         return sum(int(np.prod(g.spec.n)) * g.plan.batch for g in self.groups.values())
 
-    def autotune(self, reps=3):
+    def autotune(self, reps=5):
         """Choose every group's launch plans by timing (Plan.autotune); returns
         {group key: {pass name: chosen ms}}."""
         self._tune = False

@@ -268,7 +268,7 @@ class Plan:
         t = (C.c_int32 * n)(*[int(tiles.get(nm, 0)) for nm in _lib.PASS_NAMES])
         call("pf_set_tiles", self._h, t)
 
-    def autotune(self, u_hat, reps=3):
+    def autotune(self, u_hat, reps=5):
         """Time every launch variant of every nonlinear pass (and the fused
         against the unfused stage pass) on a copy of u_hat, keep the fastest;
         returns {pass name: chosen average ms}."""

@@ -262,7 +262,7 @@ def test_launch_variants_bit_identical(pf, N, dealias, integ):
     plan.step(ref_step, 2)
     tried = 0
     for name in TUNABLE:
-        for v in (1, 2):
+        for v in (1, 2, 3, 4):
             try:
                 plan.set_tiles({name: v})
             except NotImplementedError:
