@@ -223,10 +223,13 @@ def main():
     value = N**3 * args.steps / sec / 1e6
     fused = "fwd_axis0_stage" in {k for k, (ms, c) in prof.items() if c}
     pb = perfmodel.pass_bytes([N] * 3, "2/3", "fp32", fused=fused)
+    # per right-hand-side evaluation (4 per RK4 step): a pass may be several
+    # launches (velocity and vorticity)
+    n_eval = 4 * args.steps
     per_pass = {}
     for name, (ms, cnt) in prof.items():
         if cnt:
-            avg = ms / cnt
+            avg = ms / n_eval
             per_pass[name] = {
                 "avg_us": round(avg * 1e3, 2),
                 "launches": cnt,
@@ -234,7 +237,7 @@ def main():
                 "GBps": round(pb.get(name, 0) / (avg * 1e-3) / 1e9, 1) if name in pb else None,
             }
     dom = max((k for k in prof if k in pb), key=lambda k: prof[k][0])
-    dom_avg_s = prof[dom][0] / prof[dom][1] * 1e-3
+    dom_avg_s = prof[dom][0] / n_eval * 1e-3
     achieved = pb[dom] / dom_avg_s / 1e9
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic.json")
