@@ -180,6 +180,9 @@ int pf_accumulate_sk(pf_plan* plan, const void* u_hat, double* acc, void* stream
 #define PF_NPASS 8
 int pf_profile_enable(pf_plan* plan, int32_t enable);
 int pf_profile_read(pf_plan* plan, double* total_ms, int64_t* count, int32_t n);
+/* Passes (bit 1 << PF_PASS_*) that record events while profiling is on;
+ * default all.  Each recorded pass adds two event records to its stream. */
+int pf_profile_select(pf_plan* plan, uint32_t pass_mask);
 /* 1 (default): pf_step fuses the axis-0 forward pass with the stage update
  * where the tile fits in LDS; 0: separate passes (the unfused sequence the
  * per-stage API uses).  Results agree to rounding. */

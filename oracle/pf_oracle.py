@@ -351,6 +351,58 @@ def draw_raw_noise(rs, shape):
     return w1, w2
 
 
+# ---------------------------------------------------------------- device RNG
+# The on-device noise stream has no counterpart in the reference (which draws
+# with np.random, restated by NoiseStream below).  It is Philox4x32-10
+# (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3",
+# SC'11) and the Box-Muller map; restated here so the device draws can be
+# checked value by value.  Pinned by the published known-answer vectors.
+
+_M32 = np.uint64(0xFFFFFFFF)
+
+
+def philox4x32_10(ctr, key):
+    """Philox4x32 with 10 rounds.  ctr: four arrays of uint32 values, key: two
+    uint32 values.  Returns the four output words as uint64 arrays."""
+    x0, x1, x2, x3 = (np.asarray(v, dtype=np.uint64) & _M32 for v in ctr)
+    k0, k1 = np.uint64(key[0]) & _M32, np.uint64(key[1]) & _M32
+    m0, m1 = np.uint64(0xD2511F53), np.uint64(0xCD9E8D57)
+    w0, w1 = np.uint64(0x9E3779B9), np.uint64(0xBB67AE85)
+    for _ in range(10):
+        p0 = m0 * x0
+        p1 = m1 * x2
+        x0, x1, x2, x3 = (p1 >> np.uint64(32)) ^ x1 ^ k0, p1 & _M32, (p0 >> np.uint64(32)) ^ x3 ^ k1, p0 & _M32
+        k0 = (k0 + w0) & _M32
+        k1 = (k1 + w1) & _M32
+    return x0, x1, x2, x3
+
+
+def _box_muller(a, b):
+    u1 = (a.astype(np.float64) + 0.5) * 2.0**-32
+    u2 = b.astype(np.float64) * 2.0**-32
+    r = np.sqrt(-2.0 * np.log(u1))
+    return r * np.cos(2 * np.pi * u2), r * np.sin(2 * np.pi * u2)
+
+
+def device_noise_draws(seeds, steps, nk, nc):
+    """Raw draws (B, 2, nc, nk) of one step of the on-device stream: member b,
+    entry c, mode idx use counter (idx lo, idx hi ^ c << 24, step lo, step
+    hi) and key (seed lo, seed hi); w1 = Box-Muller of words 0-1, w2 of 2-3."""
+    idx = np.arange(nk, dtype=np.uint64)
+    out = np.zeros((len(seeds), 2, nc, nk), dtype=complex)
+    for b, (seed, step) in enumerate(zip(seeds, steps)):
+        seed, step = int(seed), int(step)
+        for c in range(nc):
+            ctr = (idx & _M32, (idx >> np.uint64(32)) ^ np.uint64(c << 24),
+                   np.full(nk, step & 0xFFFFFFFF, dtype=np.uint64), np.full(nk, (step >> 32) & 0xFFFFFFFF, dtype=np.uint64))
+            r = philox4x32_10(ctr, (seed & 0xFFFFFFFF, seed >> 32))
+            g0, g1 = _box_muller(r[0], r[1])
+            g2, g3 = _box_muller(r[2], r[3])
+            out[b, 0, c] = g0 + 1j * g1
+            out[b, 1, c] = g2 + 1j * g3
+    return out
+
+
 class NoiseStream:
     """Reproducible replacement for the reference's global MT19937 stream.
 

@@ -75,6 +75,7 @@ struct pf_plan {
   std::vector<void*> owned;
   // per-pass timing with HIP events (pf_profile_enable)
   bool prof = false;
+  uint32_t prof_mask = 0xffffffffu;  // passes that record events (pf_profile_select)
   bool fuse_stage = true;   // axis-0 forward pass fused with the stage update
   int tsel[PF_NPASS] = {};  // launch variant per pass (pf_set_tiles, pf_autotune)
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[PF_NPASS];
@@ -89,7 +90,8 @@ struct PassTimer {
   hipStream_t s;
   hipEvent_t a = nullptr, b = nullptr;
   PassTimer(pf_plan* p_, int id_, hipStream_t s_) : p(p_), id(id_), s(s_) {
-    if (p && p->prof && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess) {
+    if (p && p->prof && ((p->prof_mask >> id) & 1u) && hipEventCreate(&a) == hipSuccess &&
+        hipEventCreate(&b) == hipSuccess) {
       (void)hipEventRecord(a, s);
     } else {
       a = b = nullptr;
@@ -758,6 +760,8 @@ template <typename T> int autotune_impl(pf_plan* p, const void* uhat, int reps, 
   PF_HIP(hipMemcpyAsync(tmp.ptr, uhat, sb, hipMemcpyDeviceToDevice, s));
   PF_HIP(hipMemcpyAsync(p->u0, uhat, sb, hipMemcpyDeviceToDevice, s));
   const bool prof0 = p->prof;
+  const uint32_t mask0 = p->prof_mask;
+  p->prof_mask = 0xffffffffu;
   const int tunable[] = {PF_PASS_INV0, PF_PASS_INV1, PF_PASS_ROW, PF_PASS_FWD1, PF_PASS_FWD0, PF_PASS_FWD0_STAGE};
   double best[PF_NPASS], def[PF_NPASS];
   int bestv[PF_NPASS];
@@ -781,6 +785,7 @@ template <typename T> int autotune_impl(pf_plan* p, const void* uhat, int reps, 
     if (ts >= 0.0 && (stage_ms < 0.0 || ts < stage_ms)) stage_ms = ts;
   }
   pf_profile_enable(p, prof0);
+  p->prof_mask = mask0;
   const hipError_t e = hipStreamSynchronize(s);
   if (rc) {
     for (int id : tunable) p->tsel[id] = 0;
@@ -1101,6 +1106,12 @@ int pf_profile_enable(pf_plan* p, int32_t enable) {
     v.clear();
   }
   p->prof = enable != 0;
+  return PF_OK;
+}
+
+int pf_profile_select(pf_plan* p, uint32_t pass_mask) {
+  if (!p) return fail(PF_EINVAL, "null plan");
+  p->prof_mask = pass_mask;
   return PF_OK;
 }
 

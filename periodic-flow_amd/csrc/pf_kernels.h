@@ -297,8 +297,11 @@ struct U4 { uint32_t x, y, z, w; };
 __device__ __forceinline__ U4 philox(U4 c, uint32_t k0, uint32_t k1) {
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
-    const uint32_t lo0 = 0xD2511F53u * c.x, hi0 = __umulhi(0xD2511F53u, c.x);
-    const uint32_t lo1 = 0xCD9E8D57u * c.z, hi1 = __umulhi(0xCD9E8D57u, c.z);
+    // one 32 x 32 -> 64-bit multiply (v_mad_u64_u32) gives both halves
+    const unsigned long long p0 = (unsigned long long)0xD2511F53u * c.x;
+    const unsigned long long p1 = (unsigned long long)0xCD9E8D57u * c.z;
+    const uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
+    const uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
     c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
     k0 += 0x9E3779B9u;
     k1 += 0xBB67AE85u;
@@ -678,15 +681,30 @@ __device__ __forceinline__ void stage_fetch(const StageArgs& a, long long idx, i
 #pragma unroll
   for (int i = 0; i < D; ++i) {
     u[i] = ((const cplx<T>*)a.uhat)[mb + i * a.nk];
-    uz[i] = a.save_u0 ? mk<T>(0, 0) : ((const cplx<T>*)a.u0)[mb + i * a.nk];
+    uz[i] = (a.save_u0 || a.mode == 1) ? mk<T>(0, 0) : ((const cplx<T>*)a.u0)[mb + i * a.nk];
   }
 }
 
-template <typename T, int D>
+// MPT modes per thread (block-strided, so each load instruction stays
+// coalesced): their u_hat / u0 loads are all issued first and their noise
+// chains interleave, which hides both memory and VALU latency.
+template <typename T, int D, int MPT>
 __global__ void __launch_bounds__(kThreads) stage_kernel(StageArgs a) {
-  const long long idx = (long long)blockIdx.x * kThreads + threadIdx.x;
-  if (idx >= a.nk) return;
-  stage_point<T, D>(a, idx, blockIdx.y, nullptr);
+  const long long base = (long long)blockIdx.x * kThreads * MPT + threadIdx.x;
+  const int b = blockIdx.y;
+  cplx<T> uu[MPT][D], zz[MPT][D];
+#pragma unroll
+  for (int q = 0; q < MPT; ++q) {
+    const long long idx = base + (long long)q * kThreads;
+    if (idx < a.nk && a.mode != 2) stage_fetch<T, D>(a, idx, b, uu[q], zz[q]);
+  }
+#pragma unroll
+  for (int q = 0; q < MPT; ++q) {
+    const long long idx = base + (long long)q * kThreads;
+    if (idx >= a.nk) continue;
+    if (a.mode == 2) stage_point<T, D>(a, idx, b, nullptr);
+    else stage_point<T, D>(a, idx, b, nullptr, uu[q], zz[q]);
+  }
 }
 
 // ------------------------------------------------------------- reductions (FP64)

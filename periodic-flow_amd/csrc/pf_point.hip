@@ -39,9 +39,18 @@ static dim3 mode_grid(long long nk, int nmem) {
   return dim3((unsigned)((nk + kThreads - 1) / kThreads), (unsigned)nmem);
 }
 
+// Modes per thread: four in 2D when the launch has enough modes to fill
+// the GPU several times over (the noise chains of Stokes flow then
+// interleave), else one; one in 3D (where four would halve the occupancy).
 template <typename T> int launch_stage(int dim, const StageArgs& a, hipStream_t s) {
-  if (dim == 3) hipLaunchKernelGGL((stage_kernel<T, 3>), mode_grid(a.nk, a.n_members), dim3(kThreads), 0, s, a);
-  else hipLaunchKernelGGL((stage_kernel<T, 2>), mode_grid(a.nk, a.n_members), dim3(kThreads), 0, s, a);
+  auto grid = [&](int mpt) {
+    return dim3((unsigned)((a.nk + (long long)kThreads * mpt - 1) / ((long long)kThreads * mpt)),
+                (unsigned)a.n_members);
+  };
+  const bool wide = a.nk * a.n_members >= (4ll << 20);
+  if (dim == 3) hipLaunchKernelGGL((stage_kernel<T, 3, 1>), grid(1), dim3(kThreads), 0, s, a);
+  else if (wide) hipLaunchKernelGGL((stage_kernel<T, 2, 4>), grid(4), dim3(kThreads), 0, s, a);
+  else hipLaunchKernelGGL((stage_kernel<T, 2, 1>), grid(1), dim3(kThreads), 0, s, a);
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 

@@ -83,6 +83,7 @@ _SIGS = {
     "pf_accumulate_sk": (C.c_int, [_P, _P, _P, _P]),
     "pf_profile_enable": (C.c_int, [_P, C.c_int32]),
     "pf_profile_read": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32]),
+    "pf_profile_select": (C.c_int, [_P, C.c_uint32]),
     "pf_set_fusion": (C.c_int, [_P, C.c_int32]),
     "pf_set_tiles": (C.c_int, [_P, C.POINTER(C.c_int32)]),
     "pf_get_tiles": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),

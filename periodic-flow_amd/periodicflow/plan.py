@@ -278,8 +278,18 @@ class Plan:
         call("pf_autotune", self._h, _ptr(u_hat), int(reps), ms, _stream())
         return {nm: float(ms[i]) for i, nm in enumerate(_lib.PASS_NAMES)}
 
-    def profile(self, enable=True):
-        """Start (or stop) per-pass HIP-event timing; clears earlier records."""
+    def profile(self, enable=True, passes=None):
+        """Start (or stop) per-pass HIP-event timing; clears earlier records.
+        ``passes``: names of the passes that record (default all); each
+        recorded launch adds two event records to the stream."""
+        if passes is None:
+            mask = 0xFFFFFFFF
+        else:
+            unknown = set(passes) - set(_lib.PASS_NAMES)
+            if unknown:
+                raise ValueError(f"unknown passes {sorted(unknown)}")
+            mask = sum(1 << _lib.PASS_NAMES.index(nm) for nm in passes)
+        call("pf_profile_select", self._h, mask)
         call("pf_profile_enable", self._h, int(bool(enable)))
 
     def profile_read(self):

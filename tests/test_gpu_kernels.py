@@ -184,6 +184,24 @@ def test_rng_noise_statistics(pf):
     assert torch.equal(out, out2)
 
 
+def test_device_draws_match_restated_philox(pf):
+    """FP64 device draws equal the oracle's restatement of Philox4x32-10 and
+    Box-Muller (itself pinned by the generator's known-answer vectors), for
+    every member, entry and mode, at a nonzero step counter."""
+    N = (16, 16)
+    seeds = [7, (1 << 40) + 3]
+    plan = pf.Plan(N, 0.01, precision="fp64", nonlinear=False, batch=2, noise_mag=1.0, seeds=seeds)
+    plan.step_counter = 5
+    out = torch.empty(plan.noise_shape(), dtype=plan.complex_dtype, device="cuda")
+    plan.draw_noise(out)
+    nk = int(np.prod(plan.shape_fourier))
+    want = po.device_noise_draws(seeds, [5, 5], nk, plan.noise_shape()[2])
+    got = _host(out).reshape(want.shape)
+    err = np.abs(got - want).max()
+    _say("device draws vs restated Philox: max abs err", err)
+    assert err < 1e-12
+
+
 def test_rng_fp32_draws_match_fp64(pf):
     """The same Philox integers feed both precisions: the FP32 Box-Muller map
     (hardware log2, sin and cos) agrees with the FP64 library map to FP32

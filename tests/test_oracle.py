@@ -306,3 +306,31 @@ def test_steps_from_end_time():
     assert po.steps_from_end_time(1.25, 2.5e-3) == 500
     with pytest.raises(ValueError):
         po.steps_from_end_time(1.0, 0.3)
+
+
+# Philox4x32-10 known-answer vectors (Random123's kat_vectors, the generator's
+# reference implementation): (counter, key) -> output
+PHILOX_KAT = [
+    ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+    ((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2, (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+    ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+     (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+]
+
+
+@pytest.mark.parametrize("ctr,key,want", PHILOX_KAT)
+def test_philox_known_answers(ctr, key, want):
+    got = po.philox4x32_10([np.array([c], dtype=np.uint64) for c in ctr], key)
+    assert tuple(int(g[0]) for g in got) == want
+
+
+def test_device_draw_layout_is_unit_normal():
+    """The restated device stream: unit variance per real part, independent
+    members and entries, and a different step gives different draws."""
+    d = po.device_noise_draws([7, 8], [0, 0], 4096, 2)
+    x = np.concatenate([d.real.ravel(), d.imag.ravel()])
+    assert abs(x.mean()) < 0.02 and abs(x.var() - 1) < 0.03
+    assert abs(np.corrcoef(d[0, 0, 0].real, d[1, 0, 0].real)[0, 1]) < 0.05
+    assert abs(np.corrcoef(d[0, 0, 0].real, d[0, 0, 1].real)[0, 1]) < 0.05
+    e = po.device_noise_draws([7], [1], 4096, 1)
+    assert not np.allclose(e[0, 0, 0], d[0, 0, 0])
