@@ -190,8 +190,7 @@ template <typename T> int row(const pf_plan* p, int L, int mode, int v, const Ro
 
 ColArgs col_args() {
   ColArgs a{};
-  a.ng = 1;
-  a.g[0].n_outer = 1;
+  a.g.n_outer = 1;
   a.n_members = 1;
   a.n_fields = 1;
   a.scale = 1.0;
@@ -202,12 +201,11 @@ ColArgs col_args() {
 template <typename T>
 int col_plain(const pf_plan* p, int L, int kind, ColArgs a, int cols, int n_outer, long long in_outer,
               long long out_outer, hipStream_t s) {
-  ColGeom& g = a.g[0];
+  ColGeom& g = a.g;
   g.n2c = g.cload = g.cvalid = g.cols = cols;
   g.n_outer = n_outer;
   g.in_outer = in_outer;
   g.out_outer = out_outer;
-  a.ng = 1;
   return launch_col<T>(L, kind, a, p->tw.at(L), s, 0);
 }
 
@@ -221,74 +219,43 @@ inline int widen(int nvalid, int n2c, int cap) { return cap > 0 ? std::max(nvali
 // (rows of the axis not transformed) of `nvalid` entries each, group strides
 // in_lp / out_lp.  Tiles cover the first `nvalid` entries rounded up to whole
 // tiles; loads and stores cover them up to the row capacities cap_in /
-// cap_out (see widen).  With `nyq` > 0 the entry l = nyq of every group forms
-// a second, narrow geometry of the same launch, so every main tile is whole
-// and 128-byte aligned.
+// cap_out (see widen).
 template <typename T>
 int col_groups(pf_plan* p, int L, int kind, int v, ColArgs a, int ngroups, long long in_lp, long long out_lp,
-               int nvalid, int nyq, hipStream_t s, int cap_in = 0, int cap_out = 0) {
+               int nvalid, hipStream_t s, int cap_in = 0, int cap_out = 0) {
   const int tc = tile_cols<T>(L, kind, v);
   if (tc <= 0) return -4;
   // the fused stage kinds address the state by one flat column index
-  if ((kind == K_FWD_STAGE3 || kind == K_FWD_STAGE2) && (ngroups != 1 || nyq > 0))
+  if ((kind == K_FWD_STAGE3 || kind == K_FWD_STAGE2) && ngroups != 1)
     return fail(PF_EINVAL, "stage pass needs one flat column group");
-  ColGeom& g = a.g[0];
+  ColGeom& g = a.g;
   g.n2c = (nvalid + tc - 1) / tc * tc;
   g.cload = widen(nvalid, g.n2c, cap_in);
   g.cvalid = widen(nvalid, g.n2c, cap_out);
   g.cols = ngroups * g.n2c;
-  g.l0 = 0;
   g.in_lp = in_lp;
   g.out_lp = out_lp;
   g.n_outer = 1;
   g.in_outer = g.out_outer = 0;
-  g.in_off = g.out_off = 0;
-  a.ng = 1;
-  if (nyq > 0) {
-    ColGeom& n = a.g[1];
-    n = g;
-    n.n2c = n.cload = n.cvalid = 1;
-    n.cols = ngroups;
-    n.l0 = nyq;
-    n.in_off = n.out_off = nyq;
-    a.ng = 2;
-  }
   return launch_col<T>(L, kind, a, p->tw.at(L), s, v);
 }
 
-// The same over the last-axis columns of per-plane matrices (the axis-1
-// passes): `n_outer` matrices of columns l; the Nyquist geometry takes the
-// planes as its groups.
+// The same over the last-axis columns l of n_outer per-plane matrices (the
+// axis-1 passes).
 template <typename T>
 int col_planes(pf_plan* p, int L, int kind, int v, ColArgs a, int n_outer, long long in_outer, long long out_outer,
-               int nvalid, int nyq, hipStream_t s, int cap_in = 0, int cap_out = 0) {
+               int nvalid, hipStream_t s, int cap_in = 0, int cap_out = 0) {
   const int tc = tile_cols<T>(L, kind, v);
   if (tc <= 0) return -4;
-  ColGeom& g = a.g[0];
+  ColGeom& g = a.g;
   g.n2c = (nvalid + tc - 1) / tc * tc;
   g.cload = widen(nvalid, g.n2c, cap_in);
   g.cvalid = widen(nvalid, g.n2c, cap_out);
   g.cols = g.n2c;
-  g.l0 = 0;
   g.in_lp = g.out_lp = 0;
   g.n_outer = n_outer;
   g.in_outer = in_outer;
   g.out_outer = out_outer;
-  g.in_off = g.out_off = 0;
-  a.ng = 1;
-  if (nyq > 0) {
-    ColGeom& n = a.g[1];
-    n = g;
-    n.n2c = n.cload = n.cvalid = 1;
-    n.cols = n_outer;
-    n.in_lp = in_outer;
-    n.out_lp = out_outer;
-    n.n_outer = 1;
-    n.in_outer = n.out_outer = 0;
-    n.l0 = nyq;
-    n.in_off = n.out_off = nyq;
-    a.ng = 2;
-  }
   return launch_col<T>(L, kind, a, p->tw.at(L), s, v);
 }
 
@@ -304,9 +271,8 @@ template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStrea
     const int M0 = p->M[0], M1 = p->M[1], M2 = p->M[2];
     const long long A_f = (long long)M0 * N1 * P, B_f = (long long)M0 * M1 * P;
     const long long D_f = (long long)M0 * N1 * nh;  // D keeps the state's row length
-    // retained last-axis entries (2/3 rule) or all but the Nyquist one (3/2 rule)
-    const int lval = pr ? N2 / 3 + 1 : N2 / 2;
-    const int nyq = pr ? 0 : N2 / 2;
+    // retained last-axis entries (2/3 rule) or all nh of them (3/2 rule)
+    const int lval = pr ? N2 / 3 + 1 : (int)nh;
     // P1: axis-0 inverse of u_hat and omega_hat, u_hat -> A (6 fields, w1)
     ColArgs a = col_args();
     a.in = uhat; a.out = p->w1;
@@ -326,13 +292,13 @@ template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStrea
     }
     {
       PassTimer t(p, PF_PASS_INV0, s);
-      // 2/3 rule: the entries past N2/3 load as zero (the mask) and are
-      // stored as zeros up to whole tiles of A's padded rows
-      const int cap = pr ? (int)P : 0;
+      // the entries past lval are not loaded (zero) and are stored as zeros
+      // up to whole tiles of A's padded rows
+      const int cap = (int)P;
       a.f_base = 0;
-      PF_TRY(col_groups<T>(p, M0, K_INV_U3, p->tsel[PF_PASS_INV0], a, ngroups, nh, P, lval, nyq, s, 0, cap));
+      PF_TRY(col_groups<T>(p, M0, K_INV_U3, p->tsel[PF_PASS_INV0], a, ngroups, nh, P, lval, s, 0, cap));
       a.f_base = 3;
-      PF_TRY(col_groups<T>(p, M0, K_INV_CURL3, p->tsel[PF_PASS_INV0], a, ngroups, nh, P, lval, nyq, s, 0, cap));
+      PF_TRY(col_groups<T>(p, M0, K_INV_CURL3, p->tsel[PF_PASS_INV0], a, ngroups, nh, P, lval, s, 0, cap));
     }
     // P2: axis-1 inverse, A -> Bbuf (w2)
     a = col_args();
@@ -348,10 +314,11 @@ template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStrea
     }
     {
       PassTimer t(p, PF_PASS_INV1, s);
-      // 2/3 rule: whole tiles of the padded rows; B's entries past N2/3 are
-      // not read by the last-axis pass
-      const int cap = pr ? (int)P : 0;
-      PF_TRY(col_planes<T>(p, M1, K_INV, p->tsel[PF_PASS_INV1], a, M0, (long long)N1 * P, (long long)M1 * P, lval, nyq, s, cap, cap));
+      // whole tiles of the padded rows (A's padding holds zeros); B's
+      // entries past lval are not read by the last-axis pass
+      const int cap = (int)P;
+      PF_TRY(col_planes<T>(p, M1, K_INV, p->tsel[PF_PASS_INV1], a, M0, (long long)N1 * P, (long long)M1 * P, lval,
+                           s, cap, cap));
     }
     // P3: last axis c2r, u x omega, r2c, truncate: Bbuf -> C (w1)
     RowArgs r{};
@@ -376,7 +343,8 @@ template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStrea
     {
       PassTimer t(p, PF_PASS_FWD1, s);
       // whole tiles are loaded from C's padded rows; D's rows hold nh entries
-      PF_TRY(col_planes<T>(p, M1, K_FWD, p->tsel[PF_PASS_FWD1], a, M0, (long long)M1 * P, (long long)N1 * nh, (int)nh, 0, s, (int)P, 0));
+      PF_TRY(col_planes<T>(p, M1, K_FWD, p->tsel[PF_PASS_FWD1], a, M0, (long long)M1 * P, (long long)N1 * nh, (int)nh,
+                           s, (int)P, 0));
     }
   } else {
     const int N0 = p->N[0], N1 = p->N[1];
@@ -398,10 +366,10 @@ template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStrea
       PassTimer t(p, PF_PASS_INV0, s);
       a.n_fields = 2;
       a.f_base = 0;
-      PF_TRY(col_groups<T>(p, M0, K_INV_U2, p->tsel[PF_PASS_INV0], a, 1, 0, 0, lval, 0, s));
+      PF_TRY(col_groups<T>(p, M0, K_INV_U2, p->tsel[PF_PASS_INV0], a, 1, 0, 0, lval, s));
       a.n_fields = 1;
       a.f_base = 2;
-      PF_TRY(col_groups<T>(p, M0, K_INV_CURL2, p->tsel[PF_PASS_INV0], a, 1, 0, 0, lval, 0, s));
+      PF_TRY(col_groups<T>(p, M0, K_INV_CURL2, p->tsel[PF_PASS_INV0], a, 1, 0, 0, lval, s));
     }
     // P2: fused last axis: A -> C (2 fields, w2)
     RowArgs r{};
@@ -438,13 +406,13 @@ template <typename T> int last_forward(pf_plan* p, int kind, void* out, const St
     a.in_row = a.out_row = (long long)N1 * nh;
     a.in_field = D_f; a.in_member = d * D_f;
     a.gN1 = N1; a.gnh = (int)nh;
-    return col_groups<T>(p, M0, kind, v, a, 1, 0, 0, N1 * (int)nh, 0, s);
+    return col_groups<T>(p, M0, kind, v, a, 1, 0, 0, N1 * (int)nh, s);
   }
   const long long D_f = (long long)M0 * P;
   a.in_row = P; a.out_row = nh;
   a.in_field = D_f; a.in_member = d * D_f;
   a.gN1 = 1; a.gnh = (int)nh;
-  return col_groups<T>(p, M0, kind, v, a, 1, 0, 0, (int)nh, 0, s);
+  return col_groups<T>(p, M0, kind, v, a, 1, 0, 0, (int)nh, s);
 }
 
 // Nonlinear term (unprojected) of u_hat into the plan's R buffer (w1).

@@ -899,29 +899,25 @@ enum ColKind {
   K_INV_U2 = 7,      // velocity components of a 2D field (2/3 mask)
 };
 
-// Column enumeration of one launch geometry: column cc = jj * n2c + l of
-// the tile space sits at in_off + jj * in_lp + l in the input and
-// out_off + jj * out_lp + l in the output; columns with l < cload are
-// loaded (the rest read as zero), columns with l < cvalid are stored; l0
-// is the last-axis index of l = 0.  Loads and stores may cover whole tiles
-// past the meaningful columns where the padded rows have room for them.
+// Column enumeration of a launch: column cc = jj * n2c + l of the tile
+// space sits at jj * in_lp + l in the input and jj * out_lp + l in the
+// output; columns with l < cload are loaded (the rest read as zero), columns
+// with l < cvalid are stored.  Loads and stores may cover whole tiles past
+// the meaningful columns where the padded rows have room for them.
 struct ColGeom {
   int cols;                     // enumerated columns per matrix
-  int n2c, cload, cvalid, l0;
+  int n2c, cload, cvalid;
   int n_outer;                  // matrices per field
   int n_tiles;                  // column tiles per matrix (set by the launcher)
   long long in_lp, out_lp;
   long long in_outer, out_outer;
-  long long in_off, out_off;
-  long long n_blocks;           // blocks of this geometry (set by the launcher)
 };
 
 struct ColArgs {
   const void* in;
   void* out;
   int rows_in, rows_out;        // rows read / written per matrix
-  ColGeom g[2];                 // g[1]: the narrow Nyquist geometry (ng = 2), scheduled first
-  int ng;
+  ColGeom g;
   long long in_row, out_row;    // element strides between rows
   long long in_field, out_field;
   long long in_member, out_member;
@@ -1017,9 +1013,7 @@ __global__ void __launch_bounds__(NT) col_pass(ColArgs a, const cplx<T>* __restr
 
   const int nfg = STAGE ? 1 : a.n_fields;
   long long lg = xcd_order(blockIdx.x, gridDim.x);
-  const bool nyqg = a.ng > 1 && lg < a.g[1].n_blocks;
-  const ColGeom gm = nyqg ? a.g[1] : a.g[0];
-  if (!nyqg && a.ng > 1) lg -= a.g[1].n_blocks;
+  const ColGeom& gm = a.g;
   // PIPE consecutive tiles of one (field, matrix, member) per block; the
   // tile index varies fastest, so the blocks an XCD runs together cover
   // neighbouring columns of the same rows
@@ -1031,7 +1025,7 @@ __global__ void __launch_bounds__(NT) col_pass(ColArgs a, const cplx<T>* __restr
   const int outer = (int)(lg % gm.n_outer);
   const int b = (int)(lg / gm.n_outer);
   int c0 = tb * PIPE * TC;  // first column of the tile in hand
-  const cplx<T>* in = (const cplx<T>*)a.in + gm.in_off + (long long)b * a.in_member + (long long)outer * gm.in_outer;
+  const cplx<T>* in = (const cplx<T>*)a.in + (long long)b * a.in_member + (long long)outer * gm.in_outer;
 
   // enumerated column -> (group j, column l within the group)
   auto cpos = [&](int cc, int& j, int& l) {
@@ -1104,7 +1098,6 @@ __global__ void __launch_bounds__(NT) col_pass(ColArgs a, const cplx<T>* __restr
       if constexpr (GRID) {
         const int mi = cmode(src, a.gN0);
         const int mj = GD == 3 ? cmode(j, a.gN1) : 0;
-        l += gm.l0;
         if (a.mask23 && (abs(mi) > a.gN0 / 3 || abs(mj) > a.gN1 / 3 || l > (GD == 3 ? a.gN2 : a.gN1) / 3))
           continue;
         const cplx<T>* p = in + (long long)src * a.in_row + col;
@@ -1138,7 +1131,7 @@ __global__ void __launch_bounds__(NT) col_pass(ColArgs a, const cplx<T>* __restr
         int src = pad_src(e, a.rows_in, L, fc);
         int ia, ib;
         T ka, kb;
-        curl_k(fg, src < 0 ? 0 : src, sl.j, sl.l + gm.l0, ia, ib, ka, kb);
+        curl_k(fg, src < 0 ? 0 : src, sl.j, sl.l, ia, ib, ka, kb);
         v = mul_i<T>(mk<T>(ka * ra[q].x - kb * rb[q].x, ka * ra[q].y - kb * rb[q].y));
       }
       if constexpr (INV) {
@@ -1178,7 +1171,7 @@ __global__ void __launch_bounds__(NT) col_pass(ColArgs a, const cplx<T>* __restr
   else lds_fft<T, L, -1, W, NT, true>(buf, tws, addr);
 
   if constexpr (INV) {
-    cplx<T>* out = (cplx<T>*)a.out + gm.out_off + (long long)b * a.out_member + (long long)(a.f_base + fg) * a.out_field +
+    cplx<T>* out = (cplx<T>*)a.out + (long long)b * a.out_member + (long long)(a.f_base + fg) * a.out_field +
                    (long long)outer * gm.out_outer;
     constexpr int NO = (L * TC + NT - 1) / NT;
     // TC divides the block size: one tile column per thread
@@ -1202,14 +1195,14 @@ __global__ void __launch_bounds__(NT) col_pass(ColArgs a, const cplx<T>* __restr
     };
     if constexpr (STAGE) {
       // modes of the tile: fetch u_hat and u0 of SCH modes, then update them.
-      // State index of (row o, group j, column l): (o * N1 + j) * nh + l0 + l
-      // (2D: o * nh + l0 + l).
+      // State index of (row o, flat column cc = j nh + l): o * N1 * nh + cc
+      // (2D: o * nh + cc).
       constexpr int SCH = 2;
       const long long rowlen = NFLD == 3 ? (long long)a.gN1 * a.gnh : (long long)a.gnh;
       // TC divides the block size: one tile column per thread
       const int c = threadIdx.x % TC;
       const bool store = c0 + c < gm.cvalid;
-      const long long mcol = gm.l0 + c0 + c;
+      const long long mcol = c0 + c;
       // grid indices (j, l) of the column: one division per thread
       const int jt = NFLD == 3 ? (int)((unsigned)mcol / (unsigned)a.gnh) : 0;
       const int lt = (int)mcol - jt * a.gnh;
@@ -1233,7 +1226,7 @@ __global__ void __launch_bounds__(NT) col_pass(ColArgs a, const cplx<T>* __restr
         }
       }
     } else {
-      cplx<T>* out = (cplx<T>*)a.out + gm.out_off + (long long)b * a.out_member + (long long)fg * a.out_field +
+      cplx<T>* out = (cplx<T>*)a.out + (long long)b * a.out_member + (long long)fg * a.out_field +
                      (long long)outer * gm.out_outer;
       const Slot os = slot(threadIdx.x % TC);
       const long long ocol = (long long)os.j * gm.out_lp + os.l;

@@ -155,7 +155,8 @@ def forced3d(steps=1000, members=None, graph=False):
     t = _timed(lambda: ens.run(steps, graph=graph))
     return {"members": len(members), "steps": steps, "seconds": t,
             "Mpoint_steps_per_s": ens.points_per_step() * steps / t / 1e6, "ensemble": ens,
-            "tiles": {str(k): g.plan.tiles for k, g in ens.groups.items()}, "tuning_ms": tune}
+            "tiles": {str(k): g.plan.tiles for k, g in ens.groups.items()},
+            "tuning_ms": {str(k): v for k, v in tune.items()}}
 
 
 def serving_requests(count=400, rate=20.0, seed=42, max_steps=2000, sizes=None):
