@@ -128,50 +128,55 @@ __global__ void __launch_bounds__(kThreads) row_pass(RowArgs a, const cplx<T>* _
     }
   } else {
     // stored entries kk = 0..L/2 of each packed pair (X, Y); Z_kk = X + iY and
-    // Z_{L-kk} = conj(X) + i conj(Y) (Hermitian extension)
+    // Z_{L-kk} = conj(X) + i conj(Y) (Hermitian extension).  TPS threads
+    // own each LDS slot (pair, packed field pair) and step through its
+    // entries kk = i, i + TPS, ...: the row pointers are set once per thread.
     const cplx<T>* in = (const cplx<T>*)a.in + (long long)b * a.in_member;
     constexpr int HK = L / 2 + 1;
-    constexpr int NLD = NP * NS * HK;
-    constexpr int NIT = (NLD + kThreads - 1) / kThreads;
+    constexpr int NSL = NP * NS;
+    constexpr int TPS = kThreads / NSL;
+    static_assert(TPS >= 1, "more row slots than threads");
+    constexpr int NIT = (HK + TPS - 1) / TPS;
     constexpr int CH = NIT < 8 ? NIT : 8;
     const int kmax = a.mask23 ? N / 3 : N / 2;
+    const int slot = threadIdx.x / TPS, i0 = threadIdx.x % TPS;
+    const bool active = slot < NSL;
+    const int p = slot / NS, sl = slot % NS;
+    const int pr = pair0 + p;
+    const bool rows = active && pr < npairs;
+    const cplx<T>* px = in;
+    const cplx<T>* py = in;
+    if (rows) {
+      const long long r0 = 2LL * pr, r1 = r0 + 1;
+      if constexpr (MODE == ROW_FUSED3) {
+        const int rr = sl / 3, qq = sl % 3;
+        const long long r = r0 + rr;
+        px = in + (2 * qq) * a.in_field + r * a.in_row;
+        py = in + (2 * qq + 1) * a.in_field + r * a.in_row;
+      } else if constexpr (MODE == ROW_FUSED2) {
+        if (sl == 0) { px = in + r0 * a.in_row; py = in + a.in_field + r0 * a.in_row; }
+        else if (sl == 1) { px = in + 2 * a.in_field + r0 * a.in_row; py = in + 2 * a.in_field + r1 * a.in_row; }
+        else { px = in + r1 * a.in_row; py = in + a.in_field + r1 * a.in_row; }
+      } else {  // ROW_C2R
+        const cplx<T>* fin = in + (long long)f * a.in_field;
+        px = fin + r0 * a.in_row;
+        py = fin + r1 * a.in_row;
+      }
+    }
     for (int base = 0; base < NIT; base += CH) {
       cplx<T> X[CH], Y[CH];
 #pragma unroll
       for (int q = 0; q < CH; ++q) {
-        const int t = threadIdx.x + (base + q) * kThreads;
+        const int kk = i0 + (base + q) * TPS;
         X[q] = Y[q] = mk<T>(0, 0);
-        if (base + q >= NIT || t >= NLD) continue;
-        const int p = t / (NS * HK), rem = t % (NS * HK);
-        const int sl = rem / HK, kk = rem % HK;
-        const int pr = pair0 + p;
-        if (pr >= npairs || kk > kmax) continue;
-        const long long r0 = 2LL * pr, r1 = r0 + 1;
-        const cplx<T>* px;
-        const cplx<T>* py;
-        if constexpr (MODE == ROW_FUSED3) {
-          const int rr = sl / 3, qq = sl % 3;
-          const long long r = r0 + rr;
-          px = in + (2 * qq) * a.in_field + r * a.in_row;
-          py = in + (2 * qq + 1) * a.in_field + r * a.in_row;
-        } else if constexpr (MODE == ROW_FUSED2) {
-          if (sl == 0) { px = in + r0 * a.in_row; py = in + a.in_field + r0 * a.in_row; }
-          else if (sl == 1) { px = in + 2 * a.in_field + r0 * a.in_row; py = in + 2 * a.in_field + r1 * a.in_row; }
-          else { px = in + r1 * a.in_row; py = in + a.in_field + r1 * a.in_row; }
-        } else {  // ROW_C2R
-          const cplx<T>* fin = in + (long long)f * a.in_field;
-          px = fin + r0 * a.in_row;
-          py = fin + r1 * a.in_row;
-        }
+        if (!rows || base + q >= NIT || kk >= HK || kk > kmax) continue;
         X[q] = px[kk];
         Y[q] = py[kk];
       }
 #pragma unroll
       for (int q = 0; q < CH; ++q) {
-        const int t = threadIdx.x + (base + q) * kThreads;
-        if (base + q >= NIT || t >= NLD) continue;
-        const int p = t / (NS * HK), rem = t % (NS * HK);
-        const int sl = rem / HK, kk = rem % HK;
+        const int kk = i0 + (base + q) * TPS;
+        if (!active || base + q >= NIT || kk >= HK) continue;
         cplx<T> x = X[q], y = Y[q];
         if (kk == 0) { x.y = 0; y.y = 0; }
         if (kk == N / 2) {
@@ -179,7 +184,6 @@ __global__ void __launch_bounds__(kThreads) row_pass(RowArgs a, const cplx<T>* _
           y.y = 0;
           if (L != N) { x.x *= (T)0.5; y.x *= (T)0.5; }
         }
-        const int slot = p * NS + sl;
         buf[ix(slot, kk)] = pack<T>(x, y);
         if (kk > 0 && kk < L / 2) buf[ix(slot, L - kk)] = pack<T>(conj<T>(x), conj<T>(y));
       }
@@ -252,41 +256,50 @@ __global__ void __launch_bounds__(kThreads) row_pass(RowArgs a, const cplx<T>* _
     lds_fft<T, L, -1, NP * 2, kThreads, false>(buf, tws, fx);
   }
 
-  // ---- split the packed spectra, truncate, scale, store
+  // ---- split the packed spectra, truncate, scale, store.  TPO threads own
+  // each output slot (pair, output field pair) and step through k.
   constexpr int NFW = MODE == ROW_FUSED3 ? 3 : (MODE == ROW_FUSED2 ? 2 : 1);
+  constexpr int NOS = NP * NFW;
+  constexpr int TPO = kThreads / NOS;
+  constexpr int NKO = (L / 2 + 1 + TPO - 1) / TPO;  // nh <= L/2 + 1
   const int nh = a.nh;
   const T s = (T)a.scale;
+  const int osl = threadIdx.x / TPO, k0 = threadIdx.x % TPO;
+  const int op = osl / NFW, w = osl % NFW;
+  const int opr = pair0 + op;
+  if (osl >= NOS || opr >= npairs) return;
+  int slot;
+  if (MODE == ROW_FUSED3) slot = op * 6 + (w == 0 ? 0 : (w == 1 ? 3 : 1));
+  else if (MODE == ROW_FUSED2) slot = op * 3 + 2 * w;
+  else slot = op;
+  const long long r0 = 2LL * opr, r1 = r0 + 1;
   cplx<T>* out = (cplx<T>*)a.out + (long long)b * a.out_member;
-  for (int t = threadIdx.x; t < NP * NFW * nh; t += kThreads) {
-    const int p = t / (NFW * nh), rem = t % (NFW * nh);
-    const int w = rem / nh, k = rem % nh;
-    const int pr = pair0 + p;
-    if (pr >= npairs) continue;
-    int slot;
-    if (MODE == ROW_FUSED3) slot = p * 6 + (w == 0 ? 0 : (w == 1 ? 3 : 1));
-    else if (MODE == ROW_FUSED2) slot = p * 3 + 2 * w;
-    else slot = p;
+  cplx<T>* oa;
+  cplx<T>* ob;
+  if constexpr (MODE == ROW_FUSED3) {
+    if (w == 0) { oa = out + r0 * a.out_row; ob = out + a.out_field + r0 * a.out_row; }
+    else if (w == 1) { oa = out + r1 * a.out_row; ob = out + a.out_field + r1 * a.out_row; }
+    else { oa = out + 2 * a.out_field + r0 * a.out_row; ob = out + 2 * a.out_field + r1 * a.out_row; }
+  } else if constexpr (MODE == ROW_FUSED2) {
+    const long long r = w == 0 ? r0 : r1;
+    oa = out + r * a.out_row;
+    ob = out + a.out_field + r * a.out_row;
+  } else {
+    cplx<T>* fo = out + (long long)f * a.out_field;
+    oa = fo + r0 * a.out_row;
+    ob = fo + r1 * a.out_row;
+  }
+#pragma unroll
+  for (int q = 0; q < NKO; ++q) {
+    const int k = k0 + q * TPO;
+    if (k >= nh) break;
     const cplx<T> yk = buf[ix(slot, k)];
     const cplx<T> ym = buf[ix(slot, k == 0 ? 0 : L - k)];
     cplx<T> A = mk<T>((T)0.5 * (yk.x + ym.x), (T)0.5 * (yk.y - ym.y));
     cplx<T> B = mk<T>((T)0.5 * (yk.y + ym.y), (T)-0.5 * (yk.x - ym.x));
     if (L != N && k == N / 2) { A = mk<T>(2 * A.x, 0); B = mk<T>(2 * B.x, 0); }
-    A = scl<T>(A, s);
-    B = scl<T>(B, s);
-    const long long r0 = 2LL * pr, r1 = r0 + 1;
-    if constexpr (MODE == ROW_FUSED3) {
-      if (w == 0) { out[r0 * a.out_row + k] = A; out[a.out_field + r0 * a.out_row + k] = B; }
-      else if (w == 1) { out[r1 * a.out_row + k] = A; out[a.out_field + r1 * a.out_row + k] = B; }
-      else { out[2 * a.out_field + r0 * a.out_row + k] = A; out[2 * a.out_field + r1 * a.out_row + k] = B; }
-    } else if constexpr (MODE == ROW_FUSED2) {
-      const long long r = w == 0 ? r0 : r1;
-      out[r * a.out_row + k] = A;
-      out[a.out_field + r * a.out_row + k] = B;
-    } else {
-      cplx<T>* fo = out + (long long)f * a.out_field;
-      fo[r0 * a.out_row + k] = A;
-      fo[r1 * a.out_row + k] = B;
-    }
+    oa[k] = scl<T>(A, s);
+    ob[k] = scl<T>(B, s);
   }
 }
 
