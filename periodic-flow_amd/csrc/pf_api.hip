@@ -52,6 +52,7 @@ struct pf_plan {
   int M[3] = {1, 1, 1};
   int nh = 0;
   long long pitch = 0;           // row pitch of the work buffers (128-byte multiple)
+  long long dplane = 0;          // 3D: stride of D's axis-0 planes, N1 nh + one 128-byte line
   long long nk = 0, nphys = 0;
   int B = 1;
   bool fp64 = false;
@@ -270,7 +271,7 @@ template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStrea
     const int N0 = p->N[0], N1 = p->N[1], N2 = p->N[2];
     const int M0 = p->M[0], M1 = p->M[1], M2 = p->M[2];
     const long long A_f = (long long)M0 * N1 * P, B_f = (long long)M0 * M1 * P;
-    const long long D_f = (long long)M0 * N1 * nh;  // D keeps the state's row length
+    const long long D_f = (long long)M0 * p->dplane;  // D keeps the state's row length
     // retained last-axis entries (2/3 rule) or all nh of them (3/2 rule)
     const int lval = pr ? N2 / 3 + 1 : (int)nh;
     // P1: axis-0 inverse of u_hat and omega_hat, u_hat -> A (6 fields, w1)
@@ -343,7 +344,7 @@ template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStrea
     {
       PassTimer t(p, PF_PASS_FWD1, s);
       // whole tiles are loaded from C's padded rows; D's rows hold nh entries
-      PF_TRY(col_planes<T>(p, M1, K_FWD, p->tsel[PF_PASS_FWD1], a, M0, (long long)M1 * P, (long long)N1 * nh, (int)nh,
+      PF_TRY(col_planes<T>(p, M1, K_FWD, p->tsel[PF_PASS_FWD1], a, M0, (long long)M1 * P, p->dplane, (int)nh,
                            s, (int)P, 0));
     }
   } else {
@@ -400,10 +401,12 @@ template <typename T> int last_forward(pf_plan* p, int kind, void* out, const St
   a.out_field = p->nk; a.out_member = d * p->nk;
   if (st) a.st = *st;
   if (d == 3) {
-    // D and the state share one layout: one flat group of N1 * nh columns
+    // D and the state share one row layout: one flat group of N1 * nh
+    // columns (D's planes are one cache line further apart, which keeps the
+    // tile's rows off a near-power-of-two stride)
     const int N1 = p->N[1];
-    const long long D_f = (long long)M0 * N1 * nh;
-    a.in_row = a.out_row = (long long)N1 * nh;
+    const long long D_f = (long long)M0 * p->dplane;
+    a.in_row = p->dplane; a.out_row = (long long)N1 * nh;
     a.in_field = D_f; a.in_member = d * D_f;
     a.gN1 = N1; a.gnh = (int)nh;
     return col_groups<T>(p, M0, kind, v, a, 1, 0, 0, N1 * (int)nh, s);
@@ -835,12 +838,13 @@ int pf_plan_create(const pf_desc* desc, pf_plan** out) {
   const long long align = 128 / (long long)p->csz;
   p->pitch = (nh + align - 1) / align * align;
   const long long P = p->pitch;
+  p->dplane = d.dim == 3 ? (long long)p->N[1] * nh + align : 0;
   long long w1c = B * d.dim * p->nk, w2c = 0;  // complex elements
   if (d.nonlinear) {
     if (d.dim == 3) {
       const long long M0 = p->M[0], M1 = p->M[1], N0 = p->N[0], N1 = p->N[1];
       const long long A = B * 6 * M0 * N1 * P, Bb = B * 6 * M0 * M1 * P, C = B * 3 * M0 * M1 * P;
-      const long long D = B * 3 * M0 * N1 * nh, R = B * 3 * N0 * N1 * nh;
+      const long long D = B * 3 * M0 * p->dplane, R = B * 3 * N0 * N1 * nh;
       w1c = std::max(std::max(w1c, A), std::max(C, R));
       w2c = std::max(Bb, D);
     } else {

@@ -7,7 +7,7 @@
 //     A  (F, M0, N1, P)    axis-0 inverse, padded       <- col_pass U3 + CURL3
 //     B  (F, M0, M1, P)    axis-1 inverse, padded       <- col_pass
 //     C  (3, M0, M1, P)    c2r, u x omega, r2c, cut     <- row_pass FUSED3
-//     D  (3, M0, N1, nh)   axis-1 forward, truncated    <- col_pass
+//     D  (3, M0, N1, nh)   axis-1 forward, truncated    <- col_pass (planes 128 B apart)
 //     R  (3, N0, N1, nh)   axis-0 forward, truncated    <- col_pass (or fused stage)
 //   2D: A (3, M0, P) -> C (2, M0, P) -> R (2, N0, nh)
 // D has the state's layout so that the axis-0 forward pass walks one flat
