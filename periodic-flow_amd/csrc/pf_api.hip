@@ -53,6 +53,7 @@ struct pf_plan {
   int nh = 0;
   long long pitch = 0;           // row pitch of the work buffers (128-byte multiple)
   long long dplane = 0;          // 3D: stride of D's axis-0 planes, N1 nh + one 128-byte line
+  long long aplane = 0;          // 3D: stride of A's axis-0 planes, N1 P + one 128-byte line
   long long nk = 0, nphys = 0;
   int B = 1;
   bool fp64 = false;
@@ -270,7 +271,7 @@ template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStrea
   if (p->dim == 3) {
     const int N0 = p->N[0], N1 = p->N[1], N2 = p->N[2];
     const int M0 = p->M[0], M1 = p->M[1], M2 = p->M[2];
-    const long long A_f = (long long)M0 * N1 * P, B_f = (long long)M0 * M1 * P;
+    const long long A_f = (long long)M0 * p->aplane, B_f = (long long)M0 * M1 * P;
     const long long D_f = (long long)M0 * p->dplane;  // D keeps the state's row length
     // retained last-axis entries (2/3 rule) or all nh of them (3/2 rule)
     const int lval = pr ? N2 / 3 + 1 : (int)nh;
@@ -278,7 +279,7 @@ template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStrea
     ColArgs a = col_args();
     a.in = uhat; a.out = p->w1;
     a.rows_in = N0; a.rows_out = M0;
-    a.in_row = (long long)N1 * nh; a.out_row = (long long)N1 * P;
+    a.in_row = (long long)N1 * nh; a.out_row = p->aplane;
     a.in_field = p->nk; a.out_field = A_f;
     a.in_member = 3 * p->nk; a.out_member = 6 * A_f;
     a.n_fields = 3; a.n_members = B;
@@ -318,7 +319,7 @@ template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStrea
       // whole tiles of the padded rows (A's padding holds zeros); B's
       // entries past lval are not read by the last-axis pass
       const int cap = (int)P;
-      PF_TRY(col_planes<T>(p, M1, K_INV, p->tsel[PF_PASS_INV1], a, M0, (long long)N1 * P, (long long)M1 * P, lval,
+      PF_TRY(col_planes<T>(p, M1, K_INV, p->tsel[PF_PASS_INV1], a, M0, p->aplane, (long long)M1 * P, lval,
                            s, cap, cap));
     }
     // P3: last axis c2r, u x omega, r2c, truncate: Bbuf -> C (w1)
@@ -839,11 +840,12 @@ int pf_plan_create(const pf_desc* desc, pf_plan** out) {
   p->pitch = (nh + align - 1) / align * align;
   const long long P = p->pitch;
   p->dplane = d.dim == 3 ? (long long)p->N[1] * nh + align : 0;
+  p->aplane = d.dim == 3 ? (long long)p->N[1] * P + align : 0;
   long long w1c = B * d.dim * p->nk, w2c = 0;  // complex elements
   if (d.nonlinear) {
     if (d.dim == 3) {
       const long long M0 = p->M[0], M1 = p->M[1], N0 = p->N[0], N1 = p->N[1];
-      const long long A = B * 6 * M0 * N1 * P, Bb = B * 6 * M0 * M1 * P, C = B * 3 * M0 * M1 * P;
+      const long long A = B * 6 * M0 * p->aplane, Bb = B * 6 * M0 * M1 * P, C = B * 3 * M0 * M1 * P;
       const long long D = B * 3 * M0 * p->dplane, R = B * 3 * N0 * N1 * nh;
       w1c = std::max(std::max(w1c, A), std::max(C, R));
       w2c = std::max(Bb, D);
