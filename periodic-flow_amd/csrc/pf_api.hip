@@ -297,10 +297,10 @@ template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStrea
       // the entries past lval are not loaded (zero) and are stored as zeros
       // up to whole tiles of A's padded rows
       const int cap = (int)P;
+      // one launch: blocks of fields 0..2 transform u_hat, blocks of 3..5 omega_hat
       a.f_base = 0;
-      PF_TRY(col_groups<T>(p, M0, K_INV_U3, p->tsel[PF_PASS_INV0], a, ngroups, nh, P, lval, s, 0, cap));
-      a.f_base = 3;
-      PF_TRY(col_groups<T>(p, M0, K_INV_CURL3, p->tsel[PF_PASS_INV0], a, ngroups, nh, P, lval, s, 0, cap));
+      a.n_fields = 6;
+      PF_TRY(col_groups<T>(p, M0, K_INV_UC3, p->tsel[PF_PASS_INV0], a, ngroups, nh, P, lval, s, 0, cap));
     }
     // P2: axis-1 inverse, A -> Bbuf (w2)
     a = col_args();
@@ -366,12 +366,10 @@ template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStrea
     a.mask23 = pr;
     {
       PassTimer t(p, PF_PASS_INV0, s);
-      a.n_fields = 2;
+      // one launch: blocks of fields 0..1 transform u_hat, field 2 omega_hat
+      a.n_fields = 3;
       a.f_base = 0;
-      PF_TRY(col_groups<T>(p, M0, K_INV_U2, p->tsel[PF_PASS_INV0], a, 1, 0, 0, lval, s));
-      a.n_fields = 1;
-      a.f_base = 2;
-      PF_TRY(col_groups<T>(p, M0, K_INV_CURL2, p->tsel[PF_PASS_INV0], a, 1, 0, 0, lval, s));
+      PF_TRY(col_groups<T>(p, M0, K_INV_UC2, p->tsel[PF_PASS_INV0], a, 1, 0, 0, lval, s));
     }
     // P2: fused last axis: A -> C (2 fields, w2)
     RowArgs r{};
@@ -668,8 +666,7 @@ template <typename T> bool variant_ok(const pf_plan* p, int pass, int v) {
   const bool d3 = p->dim == 3;
   switch (pass) {
     case PF_PASS_INV0:
-      return d3 ? tile_cols<T>(p->M[0], K_INV_U3, v) > 0 && tile_cols<T>(p->M[0], K_INV_CURL3, v) > 0
-                : tile_cols<T>(p->M[0], K_INV_U2, v) > 0 && tile_cols<T>(p->M[0], K_INV_CURL2, v) > 0;
+      return tile_cols<T>(p->M[0], d3 ? K_INV_UC3 : K_INV_UC2, v) > 0;
     case PF_PASS_INV1: return d3 && tile_cols<T>(p->M[1], K_INV, v) > 0;
     case PF_PASS_ROW: return row_pair_count<T>(p->M[p->dim - 1], d3 ? ROW_FUSED3 : ROW_FUSED2, v) > 0;
     case PF_PASS_FWD1: return d3 && tile_cols<T>(p->M[1], K_FWD, v) > 0;

@@ -4,7 +4,7 @@
 //   state u_hat         (B, d, N0, [N1,] nh)      nh = N_last/2 + 1
 //   3D nonlinear passes (per member, F = 6 for u_hat and omega_hat; rows of
 //   A, B, C padded to the pitch P >= nh, a multiple of 128 bytes):
-//     A  (F, M0, N1, P)    axis-0 inverse, padded       <- col_pass U3 + CURL3
+//     A  (F, M0, N1, P)    axis-0 inverse, padded       <- col_pass UC3
 //     B  (F, M0, M1, P)    axis-1 inverse, padded       <- col_pass
 //     C  (3, M0, M1, P)    c2r, u x omega, r2c, cut     <- row_pass FUSED3
 //     D  (3, M0, N1, nh)   axis-1 forward, truncated    <- col_pass (planes 128 B apart)
@@ -892,8 +892,8 @@ __global__ void __launch_bounds__(kThreads) spectrum_kernel(DiagArgs a, int nshe
 // columns into LDS (coalesced row segments), transforms every column and
 // writes the tile back.  Kinds:
 //   K_INV        inverse, rows_in -> L with 3/2 zero padding (Nyquist split)
-//   K_INV_CURL3  axis 0 of 3D: emits u_hat and omega_hat = i k x u_hat
-//   K_INV_CURL2  axis 0 of 2D: emits u_hat and the scalar vorticity
+//   K_INV_UC3    axis 0 of 3D: emits u_hat and omega_hat = i k x u_hat
+//   K_INV_UC2    axis 0 of 2D: emits u_hat and the scalar vorticity
 //   K_FWD        forward, L -> rows_out with truncation (Nyquist sum)
 //   K_FWD_STAGE3/2  forward axis 0 of all components of a member, then the
 //                stage update of every mode of the tile (no R buffer)
@@ -903,13 +903,11 @@ __global__ void __launch_bounds__(kThreads) spectrum_kernel(DiagArgs a, int nshe
 // columns it skips are never read downstream.
 enum ColKind {
   K_INV = 0,
-  K_INV_CURL3 = 1,   // vorticity components f_base + fg, fg = 0..2, of a 3D field
-  K_INV_CURL2 = 2,   // scalar vorticity of a 2D field
   K_FWD = 3,
   K_FWD_STAGE3 = 4,
   K_FWD_STAGE2 = 5,
-  K_INV_U3 = 6,      // velocity components of a 3D field (2/3 mask, compact columns)
-  K_INV_U2 = 7,      // velocity components of a 2D field (2/3 mask)
+  K_INV_UC3 = 8,     // u_hat -> velocity (fg 0..2) and vorticity (fg 3..5) of a 3D field
+  K_INV_UC2 = 9,     // u_hat -> velocity (fg 0..1) and scalar vorticity (fg 2) of a 2D field
 };
 
 // Column enumeration of a launch: column cc = jj * n2c + l of the tile
@@ -957,14 +955,23 @@ template <typename T> constexpr int col_tc_stage(int L) {
 template <typename T, int KIND> constexpr int col_width(int L) {
   return (KIND == K_FWD_STAGE3 || KIND == K_FWD_STAGE2) ? col_tc_stage<T>(L) : col_tc<T>(L);
 }
+// Tiles per block: 2 for variant 3 of the plain kinds, whose next tile's
+// loads (at most 32 per thread, held in registers) are issued before the
+// current tile's transform and stores; otherwise 1.
+template <typename T, int KIND> constexpr int col_pipe_v(int L, int V) {
+  if (V != 3) return 1;
+  const bool kind_ok = KIND == K_INV || KIND == K_FWD;
+  const int tc = col_width<T, KIND>(L);
+  return kind_ok && tc > 0 && L * tc / kThreads <= 32 ? 2 : 1;
+}
 // Launch variants (the autotuner's choices): V = 0 the default tile width,
 // 1 half of it, 2 twice it, 3 the default width with two tiles per block,
 // 4 (stage kinds) twice the width with 1024-thread blocks; width 0 where a
-// variant does not fit in LDS or does not exist.
+// variant does not fit in LDS, does not exist or would repeat variant 0.
 template <typename T, int KIND> constexpr int col_width_v(int L, int V) {
   const int base = col_width<T, KIND>(L);
   if (base == 0 || V == 0) return base;
-  if (V == 3) return (KIND == K_FWD_STAGE3 || KIND == K_FWD_STAGE2) ? 0 : base;
+  if (V == 3) return col_pipe_v<T, KIND>(L, 3) == 2 ? base : 0;
   if (V == 4 && !(KIND == K_FWD_STAGE3 || KIND == K_FWD_STAGE2)) return 0;
   const int tc = V == 1 ? base / 2 : base * 2;
   if (tc < 1 || tc > 64) return 0;
@@ -976,15 +983,6 @@ template <typename T, int KIND> constexpr int col_width_v(int L, int V) {
 // default tile width, one block per CU holding the whole LDS), else 256.
 template <typename T, int KIND> constexpr int col_threads_v(int L, int V) {
   return (V == 4 && (KIND == K_FWD_STAGE3 || KIND == K_FWD_STAGE2)) ? 1024 : kThreads;
-}
-// Tiles per block: 2 for variant 3 of the plain and velocity kinds, whose
-// next tile's loads (at most 32 per thread, held in registers) are issued
-// before the current tile's transform and stores; otherwise 1.
-template <typename T, int KIND> constexpr int col_pipe_v(int L, int V) {
-  if (V != 3) return 1;
-  const bool kind_ok = KIND == K_INV || KIND == K_FWD || KIND == K_INV_U3 || KIND == K_INV_U2;
-  const int tc = col_width<T, KIND>(L);
-  return kind_ok && tc > 0 && L * tc / kThreads <= 32 ? 2 : 1;
 }
 
 // Row index of the padded spectrum -> source row and factor (3/2 rule,
@@ -1006,15 +1004,16 @@ __device__ __forceinline__ long long xcd_order(long long bid, long long G) {
   return x * q + (x < r ? x : r) + bid / 8;
 }
 
+// a bool as a type, to select a lambda's code path at compile time
+template <bool B> struct BoolC { static constexpr bool value = B; };
+
 template <typename T, int L, int TC, int KIND, int PIPE, int NT>
 __global__ void __launch_bounds__(NT) col_pass(ColArgs a, const cplx<T>* __restrict__ tw) {
   constexpr bool STAGE = KIND == K_FWD_STAGE3 || KIND == K_FWD_STAGE2;
-  constexpr bool INV = KIND == K_INV || KIND == K_INV_CURL3 || KIND == K_INV_CURL2 || KIND == K_INV_U3 ||
-                        KIND == K_INV_U2;
-  constexpr bool CURL = KIND == K_INV_CURL3 || KIND == K_INV_CURL2;
-  constexpr bool VEL = KIND == K_INV_U3 || KIND == K_INV_U2;
-  constexpr bool GRID = CURL || VEL;  // loads need the mode numbers of the unpadded grid
-  constexpr int GD = (KIND == K_INV_CURL3 || KIND == K_INV_U3) ? 3 : 2;
+  constexpr bool INV = KIND == K_INV || KIND == K_INV_UC3 || KIND == K_INV_UC2;
+  constexpr bool UC = KIND == K_INV_UC3 || KIND == K_INV_UC2;
+  constexpr bool GRID = UC;  // loads need the mode numbers of the unpadded grid
+  constexpr int GD = KIND == K_INV_UC3 ? 3 : 2;
   constexpr int NFLD = KIND == K_FWD_STAGE3 ? 3 : (KIND == K_FWD_STAGE2 ? 2 : 1);
   constexpr int W = NFLD * TC;  // LDS columns
   constexpr int NIT = (L * W + NT - 1) / NT;
@@ -1029,12 +1028,26 @@ __global__ void __launch_bounds__(NT) col_pass(ColArgs a, const cplx<T>* __restr
   const ColGeom& gm = a.g;
   // PIPE consecutive tiles of one (field, matrix, member) per block; the
   // tile index varies fastest, so the blocks an XCD runs together cover
-  // neighbouring columns of the same rows
+  // neighbouring columns of the same rows; the velocity + vorticity kinds
+  // take the field fastest instead, so that the blocks re-reading a tile's
+  // u_hat components for the curl run beside the one that loaded them and
+  // find them in the same XCD's L2
   const int ntb = (gm.n_tiles + PIPE - 1) / PIPE;
-  const int tb = (int)(lg % ntb);
-  lg /= ntb;
-  const int fg = (int)(lg % nfg);
-  lg /= nfg;
+  int tb, fg;
+  if constexpr (UC) {
+    fg = (int)(lg % nfg);
+    lg /= nfg;
+    tb = (int)(lg % ntb);
+    lg /= ntb;
+  } else {
+    tb = (int)(lg % ntb);
+    lg /= ntb;
+    fg = (int)(lg % nfg);
+    lg /= nfg;
+  }
+  // velocity or vorticity block (uniform over the block) and its component
+  const bool curlb = UC && fg >= GD;
+  const int cfg = (UC && fg >= GD) ? fg - GD : fg;
   const int outer = (int)(lg % gm.n_outer);
   const int b = (int)(lg / gm.n_outer);
   int c0 = tb * PIPE * TC;  // first column of the tile in hand
@@ -1090,7 +1103,10 @@ __global__ void __launch_bounds__(NT) col_pass(ColArgs a, const cplx<T>* __restr
   };
   // Loads of this thread's LDS slots base .. base + NQ - 1 of the tile at c0
   // (NQ = the register arrays' length), all issued before any is used.
-  auto issue = [&](int base, auto& ra, auto& rb) {
+  // CB: a vorticity block (a compile-time copy of curlb, so that the
+  // unrolled loops hold no per-entry branch between their loads)
+  auto issue = [&](int base, auto& ra, auto& rb, auto cb) {
+    constexpr bool CB = decltype(cb)::value;
     constexpr int NQ = (int)(sizeof(ra) / sizeof(ra[0]));
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
@@ -1114,12 +1130,12 @@ __global__ void __launch_bounds__(NT) col_pass(ColArgs a, const cplx<T>* __restr
         if (a.mask23 && (abs(mi) > a.gN0 / 3 || abs(mj) > a.gN1 / 3 || l > (GD == 3 ? a.gN2 : a.gN1) / 3))
           continue;
         const cplx<T>* p = in + (long long)src * a.in_row + col;
-        if constexpr (VEL) {
-          ra[q] = p[(long long)f * a.in_field];
+        if constexpr (!CB) {
+          ra[q] = p[(long long)cfg * a.in_field];
         } else {
           int ia, ib;
           T ka, kb;
-          curl_k(f, src, j, l, ia, ib, ka, kb);
+          curl_k(cfg, src, j, l, ia, ib, ka, kb);
           ra[q] = p[(long long)ia * a.in_field];
           rb[q] = p[(long long)ib * a.in_field];
         }
@@ -1129,14 +1145,15 @@ __global__ void __launch_bounds__(NT) col_pass(ColArgs a, const cplx<T>* __restr
     }
   };
   // ... combined (curl, 3/2-rule Nyquist halving) and stored to LDS
-  auto commit = [&](int base, const auto& ra, const auto& rb) {
+  auto commit = [&](int base, const auto& ra, const auto& rb, auto cb) {
+    constexpr bool CB = decltype(cb)::value;
     constexpr int NQ = (int)(sizeof(ra) / sizeof(ra[0]));
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const int idx = threadIdx.x + (base + q) * NT;
       if (base + q >= NIT || idx >= L * W) continue;
       cplx<T> v = ra[q];
-      if constexpr (CURL) {
+      if constexpr (CB) {
         // entries not loaded are zero in ra and rb, so any (ka, kb) serves there
         const int e = idx / W;
         const Slot sl = HOIST ? ts : slot(idx % W);
@@ -1144,7 +1161,7 @@ __global__ void __launch_bounds__(NT) col_pass(ColArgs a, const cplx<T>* __restr
         int src = pad_src(e, a.rows_in, L, fc);
         int ia, ib;
         T ka, kb;
-        curl_k(fg, src < 0 ? 0 : src, sl.j, sl.l, ia, ib, ka, kb);
+        curl_k(cfg, src < 0 ? 0 : src, sl.j, sl.l, ia, ib, ka, kb);
         v = mul_i<T>(mk<T>(ka * ra[q].x - kb * rb[q].x, ka * ra[q].y - kb * rb[q].y));
       }
       if constexpr (INV) {
@@ -1159,8 +1176,13 @@ __global__ void __launch_bounds__(NT) col_pass(ColArgs a, const cplx<T>* __restr
   };
   for (int base = 0; base < NIT; base += CH) {
     cplx<T> ra[CH], rb[CH];
-    issue(base, ra, rb);
-    commit(base, ra, rb);
+    if (UC && curlb) {
+      issue(base, ra, rb, BoolC<true>{});
+      commit(base, ra, rb, BoolC<true>{});
+    } else {
+      issue(base, ra, rb, BoolC<false>{});
+      commit(base, ra, rb, BoolC<false>{});
+    }
   }
   __syncthreads();
 
@@ -1175,7 +1197,7 @@ __global__ void __launch_bounds__(NT) col_pass(ColArgs a, const cplx<T>* __restr
     if (nxt) {
       c0 += TC;
       if constexpr (HOIST) ts = slot(threadIdx.x % W);
-      issue(0, pa, pb);
+      issue(0, pa, pb, BoolC<false>{});
       c0 -= TC;
       if constexpr (HOIST) ts = slot(threadIdx.x % W);
     }
@@ -1256,7 +1278,7 @@ __global__ void __launch_bounds__(NT) col_pass(ColArgs a, const cplx<T>* __restr
       __syncthreads();  // every read of this tile's LDS is done
       c0 += TC;
       if constexpr (HOIST) ts = slot(threadIdx.x % W);
-      commit(0, pa, pb);
+      commit(0, pa, pb, BoolC<false>{});
       __syncthreads();
     }
   }
