@@ -16,9 +16,10 @@ Differences, all additive or forced by the single-GPU design:
 * Errors raise Python exceptions instead of ``comm.Abort(1)``.
 * ``end_time`` must be a whole number of steps from the current time (the
   reference's loop would otherwise never stop, :205).
-* HDF5 output is not part of this build: with ``write_solution=True`` the
-  physical velocity of each write step is kept in ``self.snapshots``
-  (host arrays) instead of a file.
+* ``write_solution=True`` writes HDF5 snapshots through libhdf5
+  (``periodicflow.io.HDF5Writer``, the reference's writer and layouts);
+  with ``batch > 1`` member m's components are the variables ``u_m``,
+  ``v_m``[, ``w_m``].
 """
 from __future__ import annotations
 
@@ -71,7 +72,6 @@ class Solver:
         self.file_name = file_name
         self.file_mode = file_mode
         self.periodic = periodic
-        self.snapshots = []
         self.step = 0
         self.t = 0.0
         self.time_integrator = cls(dt, self.space_solver.linear_operator, optimization)
@@ -83,12 +83,33 @@ class Solver:
         self.fft_plan = fft_plan
         self._tuned = str(fft_plan).upper() == "FFTW_ESTIMATE"
         self.tuning = None
+        self.solution_writer = None
+        self._init_writer(file_name, periodic)
 
     def __getattr__(self, name):
         # simulation.py:116-120
         if name != "space_solver" and "space_solver" in self.__dict__ and hasattr(self.space_solver, name):
             return getattr(self.space_solver, name)
         raise AttributeError(f"'{type(self).__name__}' object has no attribute '{name}'")
+
+    def _init_writer(self, file_name, periodic):
+        """simulation.py:133-154: the writer reads the physical velocity
+        components, which ``backward()`` refreshes before every write."""
+        if not self.write_solution:
+            return
+        if file_name is None:
+            raise ValueError("write_solution=True needs a file_name")
+        from .io import HDF5Writer
+
+        ss = self.space_solver
+        names = ["u", "v", "w"][: ss.dim]
+        if ss.batch == 1:
+            data = {nm: [ss.u[i]] for i, nm in enumerate(names)}
+        else:
+            data = {f"{nm}_{m}": [ss.u[m, i]] for m in range(ss.batch) for i, nm in enumerate(names)}
+        self.solution_writer = HDF5Writer(None, 0, 1, file_name, solution={"space": ss, "data": data},
+                                          time_points=[0.0, self.end_time],
+                                          end_points=[x_end for _, x_end in ss.domain], periodic=periodic)
 
     @classmethod
     def read_config(cls, config: dict):
@@ -163,8 +184,9 @@ class Solver:
         self.t += n_steps * dt
 
     def _write(self):
+        # simulation.py:215-217
         self.space_solver.backward()
-        self.snapshots.append((self.step, self.t, self.space_solver.u.detach().cpu().numpy().copy()))
+        self.solution_writer.write(self.step)
 
     def solve(self):
         """simulation.py:189-225: step until end_time, with writes every
@@ -185,6 +207,9 @@ class Solver:
                 if self.verbose:
                     logger.info(f"t = {self.t:.6g}, step = {self.step}, wall = {time.perf_counter() - t0:.3f} s")
         torch.cuda.synchronize()
+        if self.write_solution:
+            # simulation.py:222-223
+            self.solution_writer.close()
         self._wall += time.perf_counter() - t0
         if self.verbose:
             logger.info(f"solve finished: {n} steps in {time.perf_counter() - t0:.3f} s")

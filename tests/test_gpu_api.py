@@ -151,16 +151,37 @@ def test_user_integrator_forward_euler(sim):
     assert po.rel_l2(_np(sol.u_hat), uh) < TOL
 
 
-def test_delegation_config_and_snapshots(sim):
+def test_delegation_config_and_snapshots(sim, tmp_path):
+    """read_config, attribute delegation, and HDF5 snapshots at step 0 and
+    every write_interval (simulation.py:179-180, :215-217), reconfigured into
+    the flat layout (io.py:142-196) and holding the physical velocity."""
+    from periodicflow import io as pio
+
+    try:
+        h5 = pio.hdf5()
+    except OSError as e:
+        pytest.skip(f"no libhdf5: {e}")
     cfg = dict(N=[8, 8], dt=0.01, end_time=0.04, time_integrator="rk2", viscosity=0.1, write_solution=True,
-               write_interval=2, seed=3)
+               file_name=str(tmp_path / "tg2d"), write_interval=2, periodic=True, seed=3)
     sol = sim.Solver.read_config(cfg)
     g = po.Grid([8, 8])
     assert np.allclose(sol.k, g.k) and np.allclose(sol.k2, g.k2) and np.allclose(sol.x[0], g.x[0])
     assert sol.dim == 2 and sol.N == [8, 8]
-    sol.initialize(po.taylor_green_2d(g))
+    u0 = po.taylor_green_2d(g)
+    sol.initialize(u0)
     sol.solve()
-    assert [s[0] for s in sol.snapshots] == [0, 2, 4]
+    u_end = _np(sol.u)
+    sol.solution_writer.reconfigure_dataset()
+    fid = h5.open(tmp_path / "tg2d.h5")
+    try:
+        assert h5.members(fid) == ["t", "u", "v", "x", "y"]
+        assert h5.members(fid, "u") == ["0", "2", "4"]
+        assert np.allclose(h5.read(fid, "t"), [0.0, 0.02, 0.04])
+        assert np.array_equal(h5.read(fid, "v/4"), np.pad(u_end[1], [(0, 1)] * 2, mode="wrap"))
+        # step 0 is the initial field (already solenoidal)
+        assert np.allclose(h5.read(fid, "u/0")[:8, :8], u0[0], rtol=0, atol=1e-13)
+    finally:
+        h5.close(fid)
     with pytest.raises(AttributeError):
         sol.no_such_attribute
 
