@@ -294,13 +294,12 @@ template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStrea
     }
     {
       PassTimer t(p, PF_PASS_INV0, s);
-      // the entries past lval are not loaded (zero) and are stored as zeros
-      // up to whole tiles of A's padded rows
-      const int cap = (int)P;
+      // the entries past lval are neither loaded nor stored: A's row padding
+      // keeps whatever it held, which only the same columns of P2 read
       // one launch: blocks of fields 0..2 transform u_hat, blocks of 3..5 omega_hat
       a.f_base = 0;
       a.n_fields = 6;
-      PF_TRY(col_groups<T>(p, M0, K_INV_UC3, p->tsel[PF_PASS_INV0], a, ngroups, nh, P, lval, s, 0, cap));
+      PF_TRY(col_groups<T>(p, M0, K_INV_UC3, p->tsel[PF_PASS_INV0], a, ngroups, nh, P, lval, s, 0, 0));
     }
     // P2: axis-1 inverse, A -> Bbuf (w2)
     a = col_args();
@@ -316,8 +315,11 @@ template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStrea
     }
     {
       PassTimer t(p, PF_PASS_INV1, s);
-      // whole tiles of the padded rows (A's padding holds zeros); B's
-      // entries past lval are not read by the last-axis pass
+      // whole tiles of the padded rows, loaded and stored (a partial last
+      // tile's stores cost more than the extra bytes: 30% at 128^3, 3/2
+      // rule).  Columns are independent, so A's padding values stay in
+      // their own columns, and B's entries past lval are not read by the
+      // last-axis pass.
       const int cap = (int)P;
       PF_TRY(col_planes<T>(p, M1, K_INV, p->tsel[PF_PASS_INV1], a, M0, p->aplane, (long long)M1 * P, lval,
                            s, cap, cap));
