@@ -293,10 +293,12 @@ class HDF5Writer:
         self._fid = self._h5.create(self.path)
 
     def write(self, step: int):
-        """io.py:42-43: each variable's current data as ``<var>/<d>D/<step>``."""
-        if self._fid is None:
-            raise RuntimeError(f"HDF5Writer: '{self.path}' is closed")
+        """io.py:42-43: each variable's current data as ``<var>/<d>D/<step>``.
+        After ``close`` (the end of a solve) the file is reopened and
+        appended to, so a later solve keeps writing into the same file."""
         h5 = self._h5
+        if self._fid is None:
+            self._fid = h5.open(self.path, write=True)
         sd = f"{self._dim}D"
         for name, fields in self.solution["data"].items():
             for field in fields:

@@ -98,8 +98,15 @@ def test_run_layout_3d(h5, tmp_path):
                 assert np.array_equal(h5.read(fid, f"{nm}/3D/{st}"), written[st][nm])
     finally:
         h5.close(fid)
-    with pytest.raises(RuntimeError):
-        w.write(200)
+    # after close (the end of a solve) a later write reopens the file and appends
+    w.write(200)
+    w.close()
+    fid = h5.open(w.path)
+    try:
+        assert sorted(h5.members(fid, "u/3D"), key=int) == ["0", "5", "10", "100", "200"]
+        assert np.array_equal(h5.read(fid, "u/3D/100"), written[100]["u"])
+    finally:
+        h5.close(fid)
 
 
 @pytest.mark.parametrize("periodic", [False, True])
