@@ -186,6 +186,34 @@ def test_delegation_config_and_snapshots(sim, tmp_path):
         sol.no_such_attribute
 
 
+def test_snapshots_batch_fp32(sim, tmp_path):
+    """With batch > 1 member m's components are the variables u_m, v_m, w_m;
+    an FP32 state is written as float64 (the reference's dtype)."""
+    from periodicflow import io as pio
+
+    try:
+        h5 = pio.hdf5()
+    except OSError as e:
+        pytest.skip(f"no libhdf5: {e}")
+    N = [8, 8, 8]
+    sol = sim.Solver(N, 0.01, 0.02, time_integrator="rk3", viscosity=0.1, precision="fp32", batch=2, seed=4,
+                     write_solution=True, file_name=str(tmp_path / "ens"), write_interval=2)
+    g = po.Grid(N)
+    u0 = np.stack([po.taylor_green_3d(g), 0.5 * po.taylor_green_3d(g)])
+    sol.initialize(u0)
+    sol.solve()
+    u_end = _np(sol.u).astype(np.float64)
+    fid = h5.open(tmp_path / "ens.h5")
+    try:
+        assert h5.members(fid) == ["u_0", "u_1", "v_0", "v_1", "w_0", "w_1"]
+        assert sorted(h5.members(fid, "v_1/3D"), key=int) == ["0", "2"]
+        assert np.array_equal(h5.read(fid, "v_1/3D/2"), u_end[1, 1])
+        assert np.array_equal(h5.read(fid, "w_0/3D/2"), u_end[0, 2])
+        assert np.allclose(h5.read(fid, "u_1/3D/0"), u0[1, 0], rtol=0, atol=1e-6)
+    finally:
+        h5.close(fid)
+
+
 def test_errors(sim):
     with pytest.raises(ValueError):
         sim.Solver([8, 8], 0.01, 0.1, time_integrator="rk7")
