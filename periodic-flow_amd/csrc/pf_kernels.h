@@ -89,7 +89,10 @@ __global__ void __launch_bounds__(kThreads) row_pass(RowArgs a, const cplx<T>* _
   constexpr int LP = row_pitch(L);
   __shared__ cplx<T> tws[tw_alloc(L)];
   __shared__ cplx<T> buf[NP * NS * LP];
-  for (int m = threadIdx.x; m < tw_size(L); m += kThreads) tws[m] = tw[m];
+  // the table size as a constant (evaluated at run time, the plan's radix
+  // array would live in private memory)
+  constexpr int TWN = tw_size(L);
+  for (int m = threadIdx.x; m < TWN; m += kThreads) tws[m] = tw[m];
 
   const int npairs = a.n_rows / 2;
   const int pair0 = blockIdx.x * NP;
@@ -966,23 +969,25 @@ template <typename T, int KIND> constexpr int col_pipe_v(int L, int V) {
 }
 // Launch variants (the autotuner's choices): V = 0 the default tile width,
 // 1 half of it, 2 twice it, 3 the default width with two tiles per block,
-// 4 (stage kinds) twice the width with 1024-thread blocks; width 0 where a
+// 4 (3D stage kind) twice the width with 1024-thread blocks; width 0 where a
 // variant does not fit in LDS, does not exist or would repeat variant 0.
 template <typename T, int KIND> constexpr int col_width_v(int L, int V) {
   const int base = col_width<T, KIND>(L);
   if (base == 0 || V == 0) return base;
   if (V == 3) return col_pipe_v<T, KIND>(L, 3) == 2 ? base : 0;
-  if (V == 4 && !(KIND == K_FWD_STAGE3 || KIND == K_FWD_STAGE2)) return 0;
+  // 1024-thread blocks cap a thread at 128 registers: the 2D stage update
+  // would spill there, so variant 4 is the 3D stage kind's only
+  if (V == 4 && KIND != K_FWD_STAGE3) return 0;
   const int tc = V == 1 ? base / 2 : base * 2;
   if (tc < 1 || tc > 64) return 0;
   const int nf = KIND == K_FWD_STAGE3 ? 3 : (KIND == K_FWD_STAGE2 ? 2 : 1);
   const long long lds = ((long long)L * nf * tc + tw_alloc(L)) * (long long)sizeof(cplx<T>);
   return lds <= 160 * 1024 ? tc : 0;
 }
-// Threads per block: 1024 for variant 4 of the fused stage kinds (twice the
+// Threads per block: 1024 for variant 4 of the 3D stage kind (twice the
 // default tile width, one block per CU holding the whole LDS), else 256.
 template <typename T, int KIND> constexpr int col_threads_v(int L, int V) {
-  return (V == 4 && (KIND == K_FWD_STAGE3 || KIND == K_FWD_STAGE2)) ? 1024 : kThreads;
+  return (V == 4 && KIND == K_FWD_STAGE3) ? 1024 : kThreads;
 }
 
 // Row index of the padded spectrum -> source row and factor (3/2 rule,
@@ -1021,7 +1026,8 @@ __global__ void __launch_bounds__(NT) col_pass(ColArgs a, const cplx<T>* __restr
   static_assert(NT % TC == 0, "tile width must divide the block size");
   __shared__ cplx<T> tws[tw_alloc(L)];
   __shared__ cplx<T> buf[L * W];
-  for (int m = threadIdx.x; m < tw_size(L); m += NT) tws[m] = tw[m];
+  constexpr int TWN = tw_size(L);  // a constant, as in row_pass
+  for (int m = threadIdx.x; m < TWN; m += NT) tws[m] = tw[m];
 
   const int nfg = STAGE ? 1 : a.n_fields;
   long long lg = xcd_order(blockIdx.x, gridDim.x);
