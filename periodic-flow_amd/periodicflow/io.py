@@ -380,7 +380,8 @@ class HDF5Writer:
         self.close()
         logger.info("HDF5Writer.reconfigure_dataset: Start reconfiguring dataset.")
         input_file = self.path
-        temp_file = Path(str(input_file).replace(".h5", "_temp.h5"))
+        # <stem>_temp.h5 beside it (io.py:155; only the file's own name changes)
+        temp_file = input_file.with_name(input_file.stem + "_temp.h5")
         if not input_file.exists():
             raise FileNotFoundError(f"Solution file '{input_file}' does not exist.")
         info = self._get_dataset_info(input_file)
