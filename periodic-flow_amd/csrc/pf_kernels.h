@@ -1180,17 +1180,24 @@ __global__ void __launch_bounds__(NT) col_pass(ColArgs a, const cplx<T>* __restr
       buf[idx] = v;  // idx = e * W + cf
     }
   };
-  for (int base = 0; base < NIT; base += CH) {
-    cplx<T> ra[CH], rb[CH];
-    if (UC && curlb) {
-      issue(base, ra, rb, BoolC<true>{});
-      commit(base, ra, rb, BoolC<true>{});
-    } else {
-      issue(base, ra, rb, BoolC<false>{});
-      commit(base, ra, rb, BoolC<false>{});
+  // The plain forward kind with one tile per block skips this staging: its
+  // first radix pass reads its inputs straight from global memory (below;
+  // 97 -> 94 us at 256^3).  The stage kinds keep it: there the direct form
+  // measured 3-7% slower at lengths 192 and 144.
+  constexpr bool DIRECT = KIND == K_FWD && PIPE == 1;
+  if constexpr (!DIRECT) {
+    for (int base = 0; base < NIT; base += CH) {
+      cplx<T> ra[CH], rb[CH];
+      if (UC && curlb) {
+        issue(base, ra, rb, BoolC<true>{});
+        commit(base, ra, rb, BoolC<true>{});
+      } else {
+        issue(base, ra, rb, BoolC<false>{});
+        commit(base, ra, rb, BoolC<false>{});
+      }
     }
+    __syncthreads();
   }
-  __syncthreads();
 
   const T s = (T)a.scale;
   auto addr = [](int ff, int e) { return e * W + ff; };
@@ -1208,8 +1215,46 @@ __global__ void __launch_bounds__(NT) col_pass(ColArgs a, const cplx<T>* __restr
       if constexpr (HOIST) ts = slot(threadIdx.x % W);
     }
   }
-  if constexpr (INV) lds_fft<T, L, +1, W, NT, true>(buf, tws, addr);
-  else lds_fft<T, L, -1, W, NT, true>(buf, tws, addr);
+  if constexpr (DIRECT) {
+    // The first Stockham pass (span 1, no twiddles) with its inputs loaded
+    // from global memory: butterfly q = (LDS column q % W, j = q / W) reads
+    // rows j + r L/R0 of its column, exactly what the staged tile would
+    // have given it, and writes its outputs to rows j R0 + r; then the
+    // remaining passes run in LDS.  One LDS round and two barriers fewer.
+    constexpr RadixPlan pl0 = make_plan(L);
+    constexpr int R0 = pl0.r[0];
+    constexpr int NB0 = L / R0;
+    constexpr int TOT0 = W * NB0;
+    constexpr int K0 = (TOT0 + NT - 1) / NT;
+    cplx<T> v0[K0][R0];
+#pragma unroll
+    for (int kk = 0; kk < K0; ++kk) {
+      const int q = threadIdx.x + kk * NT;
+      if (TOT0 % NT == 0 || q < TOT0) {
+        const Slot sl = HOIST ? ts : slot(q % W);
+        const int j = q / W;
+        const cplx<T>* pc = in + (long long)sl.f * a.in_field + (long long)sl.j * gm.in_lp + sl.l;
+#pragma unroll
+        for (int r = 0; r < R0; ++r) v0[kk][r] = sl.ld ? pc[(long long)(j + r * NB0) * a.in_row] : mk<T>(0, 0);
+      }
+    }
+#pragma unroll
+    for (int kk = 0; kk < K0; ++kk) {
+      const int q = threadIdx.x + kk * NT;
+      if (TOT0 % NT == 0 || q < TOT0) {
+        dft<R0, -1, T>(v0[kk]);
+        const int cf = q % W, j = q / W;
+#pragma unroll
+        for (int r = 0; r < R0; ++r) buf[addr(cf, j * R0 + r)] = v0[kk][r];
+      }
+    }
+    __syncthreads();
+    stockham<T, L, -1, W, NT, 1, R0, true>(buf, tws, addr);
+  } else if constexpr (INV) {
+    lds_fft<T, L, +1, W, NT, true>(buf, tws, addr);
+  } else {
+    lds_fft<T, L, -1, W, NT, true>(buf, tws, addr);
+  }
 
   if constexpr (INV) {
     cplx<T>* out = (cplx<T>*)a.out + (long long)b * a.out_member + (long long)(a.f_base + fg) * a.out_field +
