@@ -171,10 +171,11 @@ template <int S, typename T> __device__ __forceinline__ cplx<T> twiddle(const cp
 // Addr(f, e) gives the LDS index of element e of transform f.  FFAST maps
 // consecutive threads to consecutive transforms (column tiles); otherwise
 // to consecutive butterflies of one transform (contiguous rows).
-template <typename T, int L, int S, int NF, int NT, int P, int NS, bool FFAST, class Addr>
+// PEND: stop before pass PEND (the caller runs the remaining passes).
+template <typename T, int L, int S, int NF, int NT, int P, int NS, bool FFAST, class Addr, int PEND = 99>
 __device__ __forceinline__ void stockham(cplx<T>* buf, const cplx<T>* tw, const Addr& addr) {
   constexpr RadixPlan pl = make_plan(L);
-  if constexpr (P < pl.n) {
+  if constexpr (P < pl.n && P < PEND) {
     constexpr int R = pl.r[P];
     constexpr int NB = L / R;
     constexpr int TOT = NF * NB;
@@ -211,7 +212,7 @@ __device__ __forceinline__ void stockham(cplx<T>* buf, const cplx<T>* tw, const 
       }
     }
     __syncthreads();
-    stockham<T, L, S, NF, NT, P + 1, NS * R, FFAST, Addr>(buf, tw, addr);
+    stockham<T, L, S, NF, NT, P + 1, NS * R, FFAST, Addr, PEND>(buf, tw, addr);
   }
 }
 
@@ -221,6 +222,14 @@ template <typename T, int L, int S, int NF, int NT, bool FFAST, class Addr>
 __device__ __forceinline__ void lds_fft(cplx<T>* buf, const cplx<T>* tw, const Addr& addr) {
   static_assert(plan_ok(L), "FFT length must be 2^a 3^b with 1 <= a <= 11");
   stockham<T, L, S, NF, NT, 0, 1, FFAST, Addr>(buf, tw, addr);
+}
+
+// All passes but the last (the caller runs it); ends with a block barrier
+// when any pass ran.
+template <typename T, int L, int S, int NF, int NT, class Addr>
+__device__ __forceinline__ void lds_fft_head(cplx<T>* buf, const cplx<T>* tw, const Addr& addr) {
+  static_assert(plan_ok(L), "FFT length must be 2^a 3^b with 1 <= a <= 11");
+  stockham<T, L, S, NF, NT, 0, 1, true, Addr, make_plan(L).n - 1>(buf, tw, addr);
 }
 
 }  // namespace pf

@@ -1251,24 +1251,52 @@ __global__ void __launch_bounds__(NT) col_pass(ColArgs a, const cplx<T>* __restr
     __syncthreads();
     stockham<T, L, -1, W, NT, 1, R0, true>(buf, tws, addr);
   } else if constexpr (INV) {
-    lds_fft<T, L, +1, W, NT, true>(buf, tws, addr);
+    // every pass but the last in LDS; the last one stores to global memory
+    lds_fft_head<T, L, +1, W, NT>(buf, tws, addr);
   } else {
     lds_fft<T, L, -1, W, NT, true>(buf, tws, addr);
   }
 
   if constexpr (INV) {
+    // The last Stockham pass, its outputs (natural order: row j + r NS of
+    // LDS column q % W for butterfly q = (q % W, j = q / W)) scaled and
+    // stored straight to global memory: one LDS round and one barrier
+    // fewer than staging the transformed tile.
     cplx<T>* out = (cplx<T>*)a.out + (long long)b * a.out_member + (long long)(a.f_base + fg) * a.out_field +
                    (long long)outer * gm.out_outer;
-    constexpr int NO = (L * TC + NT - 1) / NT;
-    // TC divides the block size: one tile column per thread
+    constexpr RadixPlan plI = make_plan(L);
+    constexpr int PL = plI.n - 1;
+    constexpr int RL = plI.r[PL];
+    constexpr int NSL = L / RL;  // the last pass's span = its butterflies per transform
+    constexpr int TOTL = W * NSL;
+    constexpr int KL = (TOTL + NT - 1) / NT;
+    constexpr int OFFL = tw_offset(L, PL);
+    // W = TC divides the block size: every butterfly of a thread is in its column
     const Slot os = slot(threadIdx.x % TC);
     const long long ocol = (long long)os.j * gm.out_lp + os.l;
+    cplx<T> vl[KL][RL];
 #pragma unroll
-    for (int q = 0; q < NO; ++q) {
-      const int idx = threadIdx.x + q * NT;
-      if (idx >= L * TC || !os.ok) continue;
-      const int e = idx / TC;
-      out[(long long)e * a.out_row + ocol] = scl<T>(buf[e * W + os.c], s);
+    for (int kk = 0; kk < KL; ++kk) {
+      const int q = threadIdx.x + kk * NT;
+      if (TOTL % NT == 0 || q < TOTL) {
+        const int cf = q % W, j = q / W;
+#pragma unroll
+        for (int r = 0; r < RL; ++r) vl[kk][r] = buf[addr(cf, j + r * NSL)];
+        if constexpr (NSL > 1) {
+#pragma unroll
+          for (int r = 1; r < RL; ++r) vl[kk][r] = mul<T>(vl[kk][r], twiddle<+1, T>(tws, OFFL + (r - 1) * NSL + j));
+        }
+        dft<RL, +1, T>(vl[kk]);
+      }
+    }
+#pragma unroll
+    for (int kk = 0; kk < KL; ++kk) {
+      const int q = threadIdx.x + kk * NT;
+      if ((TOTL % NT == 0 || q < TOTL) && os.ok) {
+        const int j = q / W;
+#pragma unroll
+        for (int r = 0; r < RL; ++r) out[(long long)(j + r * NSL) * a.out_row + ocol] = scl<T>(vl[kk][r], s);
+      }
     }
   } else {
     const int n = a.rows_out, h = n / 2;
