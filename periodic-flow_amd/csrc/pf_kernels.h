@@ -1180,11 +1180,15 @@ __global__ void __launch_bounds__(NT) col_pass(ColArgs a, const cplx<T>* __restr
       buf[idx] = v;  // idx = e * W + cf
     }
   };
-  // The plain forward kind with one tile per block skips this staging: its
-  // first radix pass reads its inputs straight from global memory (below;
-  // 97 -> 94 us at 256^3).  The stage kinds keep it: there the direct form
-  // measured 3-7% slower at lengths 192 and 144.
-  constexpr bool DIRECT = KIND == K_FWD && PIPE == 1;
+  // The plain kinds with one tile per block skip this staging: their first
+  // radix pass reads its inputs straight from global memory (below; 97 -> 94
+  // us for the axis-1 forward at 256^3).  The stage kinds keep it (there the
+  // direct form measured 3-7% slower at lengths 192 and 144, whose first
+  // radix is 3), and so does the inverse where its first radix is below 8
+  // or the plan has a single pass (its last pass stores on its own).
+  constexpr RadixPlan plD = make_plan(L);
+  constexpr bool DIRECT =
+      PIPE == 1 && (KIND == K_FWD || (KIND == K_INV && plD.n >= 2 && plD.r[0] >= 8));
   if constexpr (!DIRECT) {
     for (int base = 0; base < NIT; base += CH) {
       cplx<T> ra[CH], rb[CH];
@@ -1221,8 +1225,7 @@ __global__ void __launch_bounds__(NT) col_pass(ColArgs a, const cplx<T>* __restr
     // rows j + r L/R0 of its column, exactly what the staged tile would
     // have given it, and writes its outputs to rows j R0 + r; then the
     // remaining passes run in LDS.  One LDS round and two barriers fewer.
-    constexpr RadixPlan pl0 = make_plan(L);
-    constexpr int R0 = pl0.r[0];
+    constexpr int R0 = plD.r[0];
     constexpr int NB0 = L / R0;
     constexpr int TOT0 = W * NB0;
     constexpr int K0 = (TOT0 + NT - 1) / NT;
@@ -1235,21 +1238,54 @@ __global__ void __launch_bounds__(NT) col_pass(ColArgs a, const cplx<T>* __restr
         const int j = q / W;
         const cplx<T>* pc = in + (long long)sl.f * a.in_field + (long long)sl.j * gm.in_lp + sl.l;
 #pragma unroll
-        for (int r = 0; r < R0; ++r) v0[kk][r] = sl.ld ? pc[(long long)(j + r * NB0) * a.in_row] : mk<T>(0, 0);
+        for (int r = 0; r < R0; ++r) {
+          const int e = j + r * NB0;
+          cplx<T> v = mk<T>(0, 0);
+          if (sl.ld) {
+            if constexpr (INV) {
+              // the inverse's row map: 3/2 padding source row (or none),
+              // the 2/3 rule's zero rows
+              double fc;
+              int src = pad_src(e, a.rows_in, L, fc);
+              if (a.prune && src >= 0 && abs(cmode(src, a.rows_in)) > a.Krow) src = -1;
+              if (src >= 0) v = pc[(long long)src * a.in_row];
+            } else {
+              v = pc[(long long)e * a.in_row];
+            }
+          }
+          v0[kk][r] = v;
+        }
       }
     }
+    constexpr int SG = INV ? +1 : -1;
 #pragma unroll
     for (int kk = 0; kk < K0; ++kk) {
       const int q = threadIdx.x + kk * NT;
       if (TOT0 % NT == 0 || q < TOT0) {
-        dft<R0, -1, T>(v0[kk]);
         const int cf = q % W, j = q / W;
+        if constexpr (INV) {
+          // the 3/2 rule halves the Nyquist row it copies to +-n/2
+          if (a.rows_in != L) {
+            const int h = a.rows_in / 2;
+#pragma unroll
+            for (int r = 0; r < R0; ++r) {
+              const int e = j + r * NB0;
+              if (e == h || e == L - h) v0[kk][r] = scl<T>(v0[kk][r], (T)0.5);
+            }
+          }
+        }
+        dft<R0, SG, T>(v0[kk]);
 #pragma unroll
         for (int r = 0; r < R0; ++r) buf[addr(cf, j * R0 + r)] = v0[kk][r];
       }
     }
     __syncthreads();
-    stockham<T, L, -1, W, NT, 1, R0, true>(buf, tws, addr);
+    if constexpr (INV) {
+      // the middle passes; the last one stores to global memory below
+      stockham<T, L, +1, W, NT, 1, R0, true, decltype(addr), plD.n - 1>(buf, tws, addr);
+    } else {
+      stockham<T, L, -1, W, NT, 1, R0, true>(buf, tws, addr);
+    }
   } else if constexpr (INV) {
     // every pass but the last in LDS; the last one stores to global memory
     lds_fft_head<T, L, +1, W, NT>(buf, tws, addr);
