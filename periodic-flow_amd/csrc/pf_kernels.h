@@ -1188,7 +1188,7 @@ __global__ void __launch_bounds__(NT) col_pass(ColArgs a, const cplx<T>* __restr
   // or the plan has a single pass (its last pass stores on its own).
   constexpr RadixPlan plD = make_plan(L);
   constexpr bool DIRECT =
-      PIPE == 1 && (KIND == K_FWD || (KIND == K_INV && plD.n >= 2 && plD.r[0] >= 8));
+      PIPE == 1 && (KIND == K_FWD || ((KIND == K_INV || UC) && plD.n >= 2 && plD.r[0] >= 8));
   if constexpr (!DIRECT) {
     for (int base = 0; base < NIT; base += CH) {
       cplx<T> ra[CH], rb[CH];
@@ -1236,12 +1236,39 @@ __global__ void __launch_bounds__(NT) col_pass(ColArgs a, const cplx<T>* __restr
       if (TOT0 % NT == 0 || q < TOT0) {
         const Slot sl = HOIST ? ts : slot(q % W);
         const int j = q / W;
-        const cplx<T>* pc = in + (long long)sl.f * a.in_field + (long long)sl.j * gm.in_lp + sl.l;
+        const cplx<T>* pcol = in + (long long)sl.j * gm.in_lp + sl.l;
+        const cplx<T>* pc = pcol + (long long)sl.f * a.in_field;
 #pragma unroll
         for (int r = 0; r < R0; ++r) {
           const int e = j + r * NB0;
           cplx<T> v = mk<T>(0, 0);
-          if (sl.ld) {
+          if constexpr (UC) {
+            // as issue + commit: the 2/3 mask on (i, j, l), then the
+            // velocity component, or the curl of two loaded components
+            // (zero operands where nothing is loaded)
+            double fc;
+            const int src = pad_src(e, a.rows_in, L, fc);
+            cplx<T> ra = mk<T>(0, 0), rb = mk<T>(0, 0);
+            int ia = 0, ib = 0;
+            T ka = 0, kb = 0;
+            if (curlb) curl_k(cfg, src < 0 ? 0 : src, sl.j, sl.l, ia, ib, ka, kb);
+            if (sl.ld && src >= 0) {
+              const int mi = cmode(src, a.gN0);
+              const int mj = GD == 3 ? cmode(sl.j, a.gN1) : 0;
+              const bool cut = a.mask23 && (abs(mi) > a.gN0 / 3 || abs(mj) > a.gN1 / 3 ||
+                                            sl.l > (GD == 3 ? a.gN2 : a.gN1) / 3);
+              if (!cut) {
+                const cplx<T>* p = pcol + (long long)src * a.in_row;
+                if (!curlb) {
+                  ra = p[(long long)cfg * a.in_field];
+                } else {
+                  ra = p[(long long)ia * a.in_field];
+                  rb = p[(long long)ib * a.in_field];
+                }
+              }
+            }
+            v = curlb ? mul_i<T>(mk<T>(ka * ra.x - kb * rb.x, ka * ra.y - kb * rb.y)) : ra;
+          } else if (sl.ld) {
             if constexpr (INV) {
               // the inverse's row map: 3/2 padding source row (or none),
               // the 2/3 rule's zero rows
