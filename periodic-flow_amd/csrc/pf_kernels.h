@@ -1184,11 +1184,11 @@ __global__ void __launch_bounds__(NT) col_pass(ColArgs a, const cplx<T>* __restr
   // radix pass reads its inputs straight from global memory (below; 97 -> 94
   // us for the axis-1 forward at 256^3).  The stage kinds keep it (there the
   // direct form measured 3-7% slower at lengths 192 and 144, whose first
-  // radix is 3), and so does the inverse where its first radix is below 8
-  // or the plan has a single pass (its last pass stores on its own).
+  // radix is 3), and so does an inverse whose plan has a single pass (its
+  // last pass stores on its own).
   constexpr RadixPlan plD = make_plan(L);
   constexpr bool DIRECT =
-      PIPE == 1 && (KIND == K_FWD || ((KIND == K_INV || UC) && plD.n >= 2 && plD.r[0] >= 8));
+      PIPE == 1 && (KIND == K_FWD || ((KIND == K_INV || UC) && plD.n >= 2));
   if constexpr (!DIRECT) {
     for (int base = 0; base < NIT; base += CH) {
       cplx<T> ra[CH], rb[CH];
