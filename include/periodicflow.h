@@ -95,6 +95,10 @@ int pf_set_seeds(pf_plan* plan, const uint64_t* seeds);
  * RNG step counter (starts at 0). */
 int pf_set_step_counter(pf_plan* plan, int64_t step);
 int64_t pf_get_step_counter(const pf_plan* plan);
+/* Adds delta to the host step counter only.  For callers that capture
+ * pf_step into a HIP graph: capture enqueues (and counts) a step that does
+ * not run, and each replay advances the device counters but not this one. */
+int pf_add_step_counter(pf_plan* plan, int64_t delta);
 /* Slot management for continuous batching (no reference counterpart):
  * the on-device noise of member m is keyed by (seed[m], its own step
  * counter), which advances on the device once per step while the member is
@@ -109,7 +113,9 @@ int pf_get_member_steps(pf_plan* plan, int64_t* out);
 /* L: device array (N0, [N1,] nh) of reals, shared by all members; NULL
  * restores -nu |k|^2.  Copied into the plan. */
 int pf_set_linear_operator(pf_plan* plan, const void* L, void* stream);
-/* C: device array (N0, [N1,] nh) of complex values.  Copied into the plan. */
+/* C: device array (N0, [N1,] nh) of complex values.  Copied into the plan.
+ * A correlated-noise plan starts with C = 0 (the reference's zero
+ * correlation_func, space_solver.py:148): its noise is zero until set. */
 int pf_set_correlation(pf_plan* plan, const void* C, void* stream);
 
 /* Unpadded transforms of `ncomp` components per member.

@@ -728,8 +728,6 @@ template <typename T> int autotune_impl(pf_plan* p, const void* uhat, int reps, 
   if (ms_out)
     for (int i = 0; i < PF_NPASS; ++i) ms_out[i] = 0.0;
   if (!p->d.nonlinear) return PF_OK;  // the step is one point-wise kernel: nothing to choose
-  if (p->d.noise_mag != 0.0 && p->d.noise_type == PF_NOISE_CORRELATED && !p->corr)
-    return fail(PF_EINVAL, "correlated noise needs pf_set_correlation first");
   const size_t sb = (size_t)p->B * p->dim * p->nk * p->csz;
   DevBuffer tmp;
   PF_HIP(hipMalloc(&tmp.ptr, sb));
@@ -873,6 +871,18 @@ int pf_plan_create(const pf_desc* desc, pf_plan** out) {
     delete p;
     return rc;
   }
+  // correlated noise: the correlation function starts at zero, as the
+  // reference's correlation_func (space_solver.py:148), so correlated noise
+  // without pf_set_correlation contributes nothing
+  if (d.noise_type == PF_NOISE_CORRELATED) {
+    hipError_t ez = hipSuccess;
+    if ((rc = dev_alloc(p, &p->corr, (size_t)p->nk * p->csz)) ||
+        (ez = hipMemset(p->corr, 0, (size_t)p->nk * p->csz)) != hipSuccess) {
+      release(p);
+      delete p;
+      return rc ? rc : hip_fail(ez, "hipMemset correlation");
+    }
+  }
   for (int i = 0; i < d.dim; ++i) {
     if ((rc = make_twiddles(p, p->N[i])) || (rc = make_twiddles(p, p->M[i]))) {
       release(p);
@@ -956,6 +966,12 @@ int pf_get_member_steps(pf_plan* p, int64_t* out) {
 
 int64_t pf_get_step_counter(const pf_plan* p) { return p ? p->step : -1; }
 
+int pf_add_step_counter(pf_plan* p, int64_t delta) {
+  if (!p) return fail(PF_EINVAL, "null plan");
+  p->step += delta;
+  return PF_OK;
+}
+
 int pf_set_linear_operator(pf_plan* p, const void* L, void* stream) {
   if (!p) return fail(PF_EINVAL, "null plan");
   if (!L) { p->lin_op = nullptr; return PF_OK; }
@@ -1035,8 +1051,6 @@ int pf_rhs_nonlinear(pf_plan* p, const void* uhat, void* rhs, void* stream) {
 int pf_step(pf_plan* p, void* uhat, int64_t n_steps, const void* noise, void* stream) {
   if (!p || !uhat || n_steps < 0) return fail(PF_EINVAL, "invalid argument");
   if (p->nact == 0 || n_steps == 0) return PF_OK;
-  if (p->d.noise_type == PF_NOISE_CORRELATED && p->d.noise_mag != 0.0 && uses_noise(p->d.integrator) && !p->corr)
-    return fail(PF_EINVAL, "correlated noise needs pf_set_correlation first");
   return p->fp64 ? step_impl<double>(p, uhat, n_steps, noise, (hipStream_t)stream)
                  : step_impl<float>(p, uhat, n_steps, noise, (hipStream_t)stream);
 }
@@ -1044,8 +1058,6 @@ int pf_step(pf_plan* p, void* uhat, int64_t n_steps, const void* noise, void* st
 int pf_compute_rhs(pf_plan* p, void* uhat, double a, const void* noise, void* rhs_nonlinear, void* rhs_linear,
                    void* forcing, void* noise_out, void* stream) {
   if (!p || !uhat || !rhs_nonlinear || !rhs_linear || !forcing || !noise_out) return fail(PF_EINVAL, "invalid argument");
-  if (p->d.noise_type == PF_NOISE_CORRELATED && p->d.noise_mag != 0.0 && !p->corr)
-    return fail(PF_EINVAL, "correlated noise needs pf_set_correlation first");
   return p->fp64 ? compute_rhs_impl<double>(p, uhat, a, noise, rhs_nonlinear, rhs_linear, forcing, noise_out,
                                             (hipStream_t)stream)
                  : compute_rhs_impl<float>(p, uhat, a, noise, rhs_nonlinear, rhs_linear, forcing, noise_out,

@@ -64,6 +64,7 @@ _SIGS = {
     "pf_set_seeds": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
     "pf_set_step_counter": (C.c_int, [_P, C.c_int64]),
     "pf_get_step_counter": (C.c_int64, [_P]),
+    "pf_add_step_counter": (C.c_int, [_P, C.c_int64]),
     "pf_set_member": (C.c_int, [_P, C.c_int32, C.c_uint64, C.c_int64]),
     "pf_set_active": (C.c_int, [_P, C.c_int32]),
     "pf_get_member_steps": (C.c_int, [_P, C.POINTER(C.c_int64)]),

@@ -73,9 +73,9 @@ class Ensemble:
                         noise_type=spec.noise_type, noise_mag=spec.noise_mag, forcing=spec.forcing,
                         forcing_power=spec.forcing_power, forcing_band=spec.forcing_band, batch=len(lst),
                         seeds=[s for _, _, s in lst])
-            if spec.noise_type == "correlated" and spec.noise_mag != 0:
-                if correlation is None:
-                    raise ValueError("correlated noise needs a correlation function: correlation(spec) -> array")
+            # without a correlation function the plan keeps C = 0 (zero
+            # noise), as the reference's zero correlation_func does
+            if spec.noise_type == "correlated" and spec.noise_mag != 0 and correlation is not None:
                 C = torch.as_tensor(np.asarray(correlation(spec)), device="cuda", dtype=plan.complex_dtype)
                 plan.set_correlation(C.contiguous())
             g = Group(spec, plan, plan.new_state(), [i for i, _, _ in lst])
@@ -123,6 +123,10 @@ class Ensemble:
         with torch.cuda.graph(g):
             for grp in self.groups.values():
                 grp.plan.step(grp.state, 1)
+        # capture records the step without running it: its host step count
+        # is taken back (the device counters did not move)
+        for grp in self.groups.values():
+            grp.plan.add_step_counter(-1)
         self._graph = g
         return g
 
@@ -130,9 +134,11 @@ class Ensemble:
         if graph:
             if self._graph is None:
                 self.capture()
-                # capture records the step without running it
             for _ in range(n_steps):
                 self._graph.replay()
+            # each replay advanced every member's device counter by one step
+            for grp in self.groups.values():
+                grp.plan.add_step_counter(n_steps)
         else:
             self.step(n_steps)
 

@@ -135,6 +135,11 @@ class Plan:
     def step_counter(self, v):
         call("pf_set_step_counter", self._h, int(v))
 
+    def add_step_counter(self, delta):
+        """Add delta to the host step counter only (graph capture / replay
+        bookkeeping; the device counters advance on the device)."""
+        call("pf_add_step_counter", self._h, int(delta))
+
     @property
     def active(self):
         return self._active

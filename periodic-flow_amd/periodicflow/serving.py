@@ -102,7 +102,7 @@ class GpuBackend:
                       integrator=spec.integrator, nonlinear=spec.nonlinear, viscosity=spec.viscosity,
                       noise_type=spec.noise_type, noise_mag=spec.noise_mag, forcing=spec.forcing,
                       forcing_power=spec.forcing_power, forcing_band=spec.forcing_band, batch=capacity)
-        if spec.noise_type == "correlated" and spec.noise_mag != 0:
+        if spec.noise_type == "correlated" and spec.noise_mag != 0 and self.correlation is not None:
             C = self.torch.as_tensor(np.asarray(self.correlation(spec)), device="cuda", dtype=p.complex_dtype)
             p.set_correlation(C.contiguous())
         p.active = 0

@@ -88,6 +88,11 @@ def test_graph_replay_matches_eager(mods):
         torch.cuda.synchronize()
         for i in range(len(members)):
             assert torch.equal(a.state(i), b.state(i)), (rep, i)
+        # host and device step counters agree with the eager run's
+        for ka, kb in zip(a.groups, b.groups):
+            pa, pb = a.groups[ka].plan, b.groups[kb].plan
+            assert pa.step_counter == pb.step_counter == 2 * (rep + 1)
+            assert pa.member_steps() == pb.member_steps()
 
 
 def test_serving_results_equal_alone(mods):
