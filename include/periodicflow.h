@@ -132,6 +132,20 @@ int pf_mask_zero_mode(pf_plan* plan, void* u_hat, void* stream);
  * applied when mask_nyquist is set).  rhs may not alias u_hat. */
 int pf_rhs_nonlinear(pf_plan* plan, const void* u_hat, void* rhs, void* stream);
 
+/* pf_rhs_nonlinear without a plan and without allocating: no counterpart in
+ * the reference; for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * desc describes the grid, batch, precision and dealiasing as for
+ * pf_plan_create (desc->nonlinear must be 1); u_hat and rhs are laid out as
+ * for a plan of that descriptor.  Twiddle tables and work buffers live in a
+ * static device array of 128 MiB per GPU (held from library load until the
+ * process ends); descriptors that need more are rejected with PF_EINVAL
+ * (FP32: up to 96^3 under the 3/2 rule, 128^3 under the 2/3 rule, 1024^2;
+ * FP64: 64^3 under the 3/2 rule).  Calls on one GPU share that array:
+ * calls that overlap in time on one GPU (other streams, other threads) race
+ * and must be ordered by the caller. */
+int pf_rhs_nonlinear_noalloc(const pf_desc* desc, const void* u_hat, void* rhs, void* stream);
+
 /* Advance u_hat by n_steps time steps in place.
  * noise == NULL: noise drawn on the device (Philox4x32-10 keyed by the
  *   member seed, the step counter, the mode and the component).

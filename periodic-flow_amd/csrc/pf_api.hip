@@ -1,8 +1,9 @@
 // C ABI: plans, workspace, and the per-stage launch sequence.
 #include <cmath>
+#include <cstdarg>
 #include <cstdio>
 #include <cstring>
-#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -13,16 +14,24 @@ using namespace pf;
 
 namespace {
 
-thread_local std::string g_err;
+// the last error of the calling thread (a fixed buffer: reporting an error
+// allocates nothing)
+thread_local char g_err[512];
 
-int fail(int code, const std::string& msg) {
-  g_err = msg;
+int fail(int code, const char* msg) {
+  snprintf(g_err, sizeof g_err, "%s", msg);
+  return code;
+}
+int fail(int code, const std::string& msg) { return fail(code, msg.c_str()); }
+__attribute__((format(printf, 2, 3))) int failf(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
   return code;
 }
 
-int hip_fail(hipError_t e, const char* what) {
-  return fail(PF_EHIP, std::string(what) + ": " + hipGetErrorString(e));
-}
+int hip_fail(hipError_t e, const char* what) { return failf(PF_EHIP, "%s: %s", what, hipGetErrorString(e)); }
 
 #define PF_HIP(call)                                   \
   do {                                                 \
@@ -37,10 +46,9 @@ int hip_fail(hipError_t e, const char* what) {
   } while (0)
 
 int launch_fail(int rc, const char* what) {
-  if (rc == -4) return fail(PF_EUNSUPPORTED, std::string("unsupported transform length in ") + what);
-  if (rc == -3) return fail(PF_EHIP, std::string("kernel launch failed: ") + what + ": " +
-                                         hipGetErrorString(hipGetLastError()));
-  return fail(PF_EINVAL, std::string("invalid launch: ") + what);
+  if (rc == -4) return failf(PF_EUNSUPPORTED, "unsupported transform length in %s", what);
+  if (rc == -3) return failf(PF_EHIP, "kernel launch failed: %s: %s", what, hipGetErrorString(hipGetLastError()));
+  return failf(PF_EINVAL, "invalid launch: %s", what);
 }
 
 }  // namespace
@@ -71,7 +79,9 @@ struct pf_plan {
   long long* msteps = nullptr;   // per-member RNG step counters (device)
   int nact = 1;                  // members [0, nact) are advanced
   double* fcoef = nullptr;
-  std::map<int, void*> tw;
+  // twiddle tables of the plan's distinct transform lengths (at most 2 per axis)
+  struct Tw { int L; void* ptr; } tw[6] = {};
+  int ntw = 0;
   long long step = 0;
   size_t bytes = 0;
   std::vector<void*> owned;
@@ -133,8 +143,16 @@ bool supported_length(int L) {
   }
 }
 
+// Twiddle table of length L (the plan holds one per distinct length).
+const void* tw_of(const pf_plan* p, int L) {
+  for (int i = 0; i < p->ntw; ++i)
+    if (p->tw[i].L == L) return p->tw[i].ptr;
+  return nullptr;
+}
+
 int make_twiddles(pf_plan* p, int L) {
-  if (p->tw.count(L)) return 0;
+  if (tw_of(p, L)) return 0;
+  if (p->ntw >= 6) return fail(PF_EINVAL, "too many transform lengths");
   void* dptr = nullptr;
   int rc = dev_alloc(p, &dptr, (size_t)L * p->csz);
   if (rc) return rc;
@@ -165,7 +183,7 @@ int make_twiddles(pf_plan* p, int L) {
     for (size_t m = 0; m < re.size(); ++m) { h[2 * m] = (float)re[m]; h[2 * m + 1] = (float)im[m]; }
     PF_HIP(hipMemcpy(dptr, h.data(), 2 * re.size() * sizeof(float), hipMemcpyHostToDevice));
   }
-  p->tw[L] = dptr;
+  p->tw[p->ntw++] = {L, dptr};
   return 0;
 }
 
@@ -187,7 +205,7 @@ StageArgs geometry(const pf_plan* p) {
 
 // v: the launch variant (0 = default; the autotuner's choice per pass)
 template <typename T> int row(const pf_plan* p, int L, int mode, int v, const RowArgs& a, hipStream_t s) {
-  return launch_row<T>(L, mode, a, p->tw.at(L), s, v);
+  return launch_row<T>(L, mode, a, tw_of(p, L), s, v);
 }
 
 ColArgs col_args() {
@@ -208,7 +226,7 @@ int col_plain(const pf_plan* p, int L, int kind, ColArgs a, int cols, int n_oute
   g.n_outer = n_outer;
   g.in_outer = in_outer;
   g.out_outer = out_outer;
-  return launch_col<T>(L, kind, a, p->tw.at(L), s, 0);
+  return launch_col<T>(L, kind, a, tw_of(p, L), s, 0);
 }
 
 // Loaded / stored column counts of a geometry: the first `nvalid` entries,
@@ -241,7 +259,7 @@ int col_groups(pf_plan* p, int L, int kind, int v, ColArgs a, int ngroups, long 
   g.out_lp = out_lp;
   g.n_outer = 1;
   g.in_outer = g.out_outer = 0;
-  return launch_col<T>(L, kind, a, p->tw.at(L), s, v);
+  return launch_col<T>(L, kind, a, tw_of(p, L), s, v);
 }
 
 // The same over the last-axis columns l of n_outer per-plane matrices (the
@@ -260,7 +278,7 @@ int col_planes(pf_plan* p, int L, int kind, int v, ColArgs a, int n_outer, long 
   g.n_outer = n_outer;
   g.in_outer = in_outer;
   g.out_outer = out_outer;
-  return launch_col<T>(L, kind, a, p->tw.at(L), s, v);
+  return launch_col<T>(L, kind, a, tw_of(p, L), s, v);
 }
 
 // Inverse passes, the fused last-axis pass and (3D) the axis-1 forward pass
@@ -787,17 +805,10 @@ template <typename T> int autotune_impl(pf_plan* p, const void* uhat, int reps, 
   return PF_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-const char* pf_last_error(void) { return g_err.c_str(); }
-const char* pf_version(void) { return "periodicflow-hip 0.1 gfx950"; }
-
-int pf_plan_create(const pf_desc* desc, pf_plan** out) {
-  if (!desc || !out) return fail(PF_EINVAL, "null argument");
-  *out = nullptr;
-  const pf_desc& d = *desc;
+// Validates a descriptor and fills the plan's geometry (grid, padded
+// lengths, pitches, strides) and the sizes of its two work buffers in
+// complex elements.  Touches no device and allocates nothing.
+int plan_geometry(pf_plan* p, const pf_desc& d, long long& w1c, long long& w2c) {
   if (d.dim != 2 && d.dim != 3) return fail(PF_EINVAL, "NS solver only supports 2D and 3D problems.");
   if (d.precision != PF_FP32 && d.precision != PF_FP64) return fail(PF_EINVAL, "precision must be PF_FP32 or PF_FP64");
   if (d.dealias != PF_DEALIAS_PAD32 && d.dealias != PF_DEALIAS_TRUNC23) return fail(PF_EINVAL, "invalid dealiasing rule");
@@ -807,7 +818,6 @@ int pf_plan_create(const pf_desc* desc, pf_plan** out) {
   if (d.forcing != PF_FORCING_NONE && d.forcing != PF_FORCING_CONSTANT_POWER) return fail(PF_EINVAL, "invalid forcing");
   if (d.batch < 1) return fail(PF_EINVAL, "batch must be >= 1");
   if (!(d.dt > 0)) return fail(PF_EINVAL, "dt must be > 0");
-  pf_plan* p = new pf_plan();
   p->d = d;
   p->dim = d.dim;
   p->B = d.batch;
@@ -817,18 +827,15 @@ int pf_plan_create(const pf_desc* desc, pf_plan** out) {
   p->direct23 = d.dealias == PF_DEALIAS_TRUNC23;
   for (int i = 0; i < d.dim; ++i) {
     const int n = d.n[i];
-    if (n < 4 || n % 2) { delete p; return fail(PF_EINVAL, "grid sizes must be even and >= 4"); }
+    if (n < 4 || n % 2) return fail(PF_EINVAL, "grid sizes must be even and >= 4");
     p->N[i] = n;
     p->M[i] = p->direct23 ? n : (3 * n) / 2;
-    if (!supported_length(p->N[i]) || !supported_length(p->M[i])) {
-      delete p;
-      return fail(PF_EUNSUPPORTED, "grid size " + std::to_string(n) + " (padded " + std::to_string(p->M[i]) +
-                                       ") is not a compiled transform length");
-    }
-    if (!(d.length[i] > 0)) { delete p; return fail(PF_EINVAL, "box lengths must be > 0"); }
+    if (!supported_length(p->N[i]) || !supported_length(p->M[i]))
+      return failf(PF_EUNSUPPORTED, "grid size %d (padded %d) is not a compiled transform length", n, p->M[i]);
+    if (!(d.length[i] > 0)) return fail(PF_EINVAL, "box lengths must be > 0");
     p->ks[i] = 6.283185307179586476925286766559 / d.length[i];
   }
-  if (p->M[d.dim - 1] % 2) { delete p; return fail(PF_EINVAL, "padded last axis must be even"); }
+  if (p->M[d.dim - 1] % 2) return fail(PF_EINVAL, "padded last axis must be even");
   const int last = d.dim - 1;
   p->nh = p->N[last] / 2 + 1;
   p->nk = (long long)p->nh;
@@ -843,7 +850,8 @@ int pf_plan_create(const pf_desc* desc, pf_plan** out) {
   const long long P = p->pitch;
   p->dplane = d.dim == 3 ? (long long)p->N[1] * nh + align : 0;
   p->aplane = d.dim == 3 ? (long long)p->N[1] * P + align : 0;
-  long long w1c = B * d.dim * p->nk, w2c = 0;  // complex elements
+  w1c = B * d.dim * p->nk;
+  w2c = 0;
   if (d.nonlinear) {
     if (d.dim == 3) {
       const long long M0 = p->M[0], M1 = p->M[1], N0 = p->N[0], N1 = p->N[1];
@@ -858,6 +866,28 @@ int pf_plan_create(const pf_desc* desc, pf_plan** out) {
       w2c = C;
     }
   }
+  p->nact = p->B;
+  return PF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pf_last_error(void) { return g_err; }
+const char* pf_version(void) { return "periodicflow-hip 0.1 gfx950"; }
+
+int pf_plan_create(const pf_desc* desc, pf_plan** out) {
+  if (!desc || !out) return fail(PF_EINVAL, "null argument");
+  *out = nullptr;
+  const pf_desc& d = *desc;
+  pf_plan* p = new pf_plan();
+  long long w1c = 0, w2c = 0;
+  if (int rc0 = plan_geometry(p, d, w1c, w2c)) {
+    delete p;
+    return rc0;
+  }
+  const long long B = p->B;
   int rc = 0;
   p->w1_bytes = (size_t)w1c * p->csz;
   p->w2_bytes = (size_t)w2c * p->csz;
@@ -1181,6 +1211,123 @@ int pf_accumulate_sk(pf_plan* p, const void* uhat, double* acc, void* stream) {
   const int rc = p->fp64 ? launch_sk<double>(p->dim, uhat, acc, p->nk, p->nact, (hipStream_t)stream)
                          : launch_sk<float>(p->dim, uhat, acc, p->nk, p->nact, (hipStream_t)stream);
   return rc ? launch_fail(rc, "accumulate_sk") : PF_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------ allocation-free entry
+// pf_rhs_nonlinear_noalloc keeps its twiddle tables and work buffers in one
+// file-scope device array instead of memory it allocates.  Each GPU holds its
+// own copy of the array (from when the library's code is loaded there until
+// the process ends).  Host code must never pass `scratch` itself to a kernel
+// or a copy: the name stands for a host-side shadow.  The device address,
+// looked up once per device with hipGetSymbolAddress, is what the kernels get.
+#define PF_SCRATCH_BYTES (128ull << 20)
+static __device__ __align__(256) unsigned char scratch[PF_SCRATCH_BYTES];
+
+namespace {
+
+constexpr size_t kTwSlot = 32 * 1024;  // bytes per twiddle table (length <= 1536 in FP64: 24 KB)
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up on its first use there).
+int scratch_address(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) return failf(PF_EINVAL, "device index %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    if (!ptr) return fail(PF_EHIP, "hipGetSymbolAddress(scratch) returned null");
+    g_scratch_dev[dev] = ptr;
+  }
+  *out = g_scratch_dev[dev];
+  return PF_OK;
+}
+
+// The twiddle table of length L (layout of make_twiddles) computed on the
+// device in double precision: entry OFF(q) + (r - 1) Ns + k of pass q >= 1 is
+// exp(-2 pi i r k / (Ns R)).
+template <typename T> __global__ void __launch_bounds__(256) twiddle_kernel(cplx<T>* out, int n, RadixPlan pl) {
+  const int m = blockIdx.x * 256 + threadIdx.x;
+  if (m >= n) return;
+  int off = 0, ns = pl.r[0];
+  for (int q = 1; q < pl.n; ++q) {
+    const int R = pl.r[q];
+    const int cnt = (R - 1) * ns;
+    if (m < off + cnt) {
+      const int e = m - off;
+      const int r = e / ns + 1, k = e % ns;
+      double sn, cs;
+      sincospi(2.0 * (double)(r * k) / (double)(ns * R), &sn, &cs);
+      out[m] = mk<T>((T)cs, (T)-sn);
+      return;
+    }
+    off += cnt;
+    ns *= R;
+  }
+}
+
+template <typename T> int noalloc_rhs(pf_plan* p, const void* uhat, void* rhs, hipStream_t s) {
+  for (int t = 0; t < p->ntw; ++t) {
+    const int L = p->tw[t].L, n = tw_size(L);
+    if (n == 0) continue;
+    hipLaunchKernelGGL(twiddle_kernel<T>, dim3((n + 255) / 256), dim3(256), 0, s, (cplx<T>*)p->tw[t].ptr, n,
+                       make_plan(L));
+    PF_HIP(hipGetLastError());
+  }
+  int rc = nonlinear_unprojected<T>(p, uhat, s);
+  if (rc) return rc;
+  const StageArgs g = geometry(p);
+  rc = launch_project<T>(p->dim, g, p->w1, rhs, p->d.mask_nyquist, 0, p->nact, s);
+  return rc ? launch_fail(rc, "project") : PF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pf_rhs_nonlinear_noalloc(const pf_desc* desc, const void* u_hat, void* rhs, void* stream) {
+  if (!desc || !u_hat || !rhs) return fail(PF_EINVAL, "invalid argument");
+  if (!desc->nonlinear) return fail(PF_EINVAL, "descriptor has nonlinear = 0");
+  pf_plan p;  // geometry only: its tables stay empty, nothing is allocated
+  long long w1c = 0, w2c = 0;
+  int rc = plan_geometry(&p, *desc, w1c, w2c);
+  if (rc) return rc;
+  int lens[6];
+  int nl = 0;
+  for (int i = 0; i < p.dim; ++i)
+    for (int L : {p.N[i], p.M[i]}) {
+      bool seen = false;
+      for (int t = 0; t < nl; ++t) seen = seen || lens[t] == L;
+      if (!seen) lens[nl++] = L;
+    }
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  p.w1_bytes = (size_t)w1c * p.csz;
+  p.w2_bytes = (size_t)w2c * p.csz;
+  const size_t tw_bytes = (size_t)nl * kTwSlot;
+  const size_t need = tw_bytes + up(p.w1_bytes) + up(p.w2_bytes);
+  if (need > PF_SCRATCH_BYTES)
+    return failf(PF_EINVAL,
+                 "pf_rhs_nonlinear_noalloc: this grid and batch need %zu bytes of work memory; the static scratch "
+                 "holds %llu (create a plan and call pf_rhs_nonlinear instead)",
+                 need, (unsigned long long)PF_SCRATCH_BYTES);
+  void* base = nullptr;
+  if ((rc = scratch_address(&base))) return rc;
+  char* b = (char*)base;
+  for (int t = 0; t < nl; ++t) {
+    if ((size_t)tw_size(lens[t]) * p.csz > kTwSlot) return failf(PF_EINVAL, "twiddle table of length %d too large", lens[t]);
+    p.tw[p.ntw++] = {lens[t], b + (size_t)t * kTwSlot};
+  }
+  p.w1 = b + tw_bytes;
+  p.w2 = w2c ? b + tw_bytes + up(p.w1_bytes) : nullptr;
+  hipStream_t s = (hipStream_t)stream;
+  return p.fp64 ? noalloc_rhs<double>(&p, u_hat, rhs, s) : noalloc_rhs<float>(&p, u_hat, rhs, s);
 }
 
 }  // extern "C"

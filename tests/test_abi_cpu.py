@@ -111,3 +111,21 @@ def test_perfmodel_matches_byte_table():
     # 3/2 rule pads the inverse outputs by 3/2 per padded axis, nothing pruned
     pb32 = perfmodel.pass_bytes([64] * 3, "3/2", "fp64")
     assert pb32["inv_axis0_curl"] == (3 * 64 * 64 * 33 + 6 * 96 * 64 * 33) * 16
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(dim=4), "2D and 3D"),
+    (dict(nonlinear=0), "nonlinear = 0"),
+    (dict(n=(256, 256, 256)), "static scratch"),
+    (dict(n=(128, 128, 128), precision=1, dealias=1), "static scratch"),
+    (dict(n=(20, 16, 16)), "transform length"),
+])
+def test_noalloc_rejects_before_touching_the_device(lib, kw, msg):
+    """The allocation-free entry point validates the descriptor and checks
+    its work memory against the static scratch before any HIP call."""
+    d = _desc(**kw)
+    x = (C.c_char * 16)()
+    rc = lib.pf_rhs_nonlinear_noalloc(C.byref(d), x, x, None)
+    assert rc in (-1, -4)
+    assert msg in lib.pf_last_error().decode()
+    assert lib.pf_rhs_nonlinear_noalloc(None, x, x, None) == -1
