@@ -155,6 +155,17 @@ int pf_rhs_nonlinear_noalloc(const pf_desc* desc, const void* u_hat, void* rhs, 
  * The step counter advances by n_steps. */
 int pf_step(pf_plan* plan, void* u_hat, int64_t n_steps, const void* noise, void* stream);
 
+/* pf_step of n_steps with the static structure factor accumulated after
+ * every step: acc (device doubles, nk) += sum over the active members and
+ * components of |u_hat|^2 (pf_accumulate_sk after each step).  Stokes flow
+ * with implicit PC, no forcing and on-device noise runs the steps with the
+ * state held in registers and acc summed in FP64 registers (one launch per
+ * 200 steps).  The loop of Solver.solve (simulation.py:205-213) with a
+ * reduction the reference lacks (its thermal-noise example writes snapshots,
+ * examples/fluctuating_stokes_thermal_noise.py:24-36, for statistics
+ * afterwards). */
+int pf_step_accumulate_sk(pf_plan* plan, void* u_hat, int64_t n_steps, double* acc, void* stream);
+
 /* compute_rhs(a) of one stage (space_solver.py:302-310): writes the
  * projected (and Nyquist-masked) nonlinear term, L u_hat, the forcing and
  * the projected noise built from w1 + a w2 into four arrays shaped like
@@ -212,8 +223,11 @@ int pf_set_fusion(pf_plan* plan, int32_t fuse_stage);
  * are planned by shenfun/FFTW: fft_plan="FFTW_MEASURE", pf/simulation.py:76,
  * pf/basis.py:63-65).  tiles[PF_NPASS], indexed by PF_PASS_*: 0 is the
  * default tile width (column passes) or row-pair count (row passes); FP32
- * plans also have 1 (half) and 2 (twice) where they fit in LDS.  A choice
- * changes only how work is mapped to threads: results are bit-identical.
+ * plans also have 1 (half) and 2 (twice) where they fit in LDS.  Plans of
+ * Stokes flow with implicit PC and no forcing have PF_PASS_STAGE variant 1:
+ * one launch per step instead of many steps with the state in registers.  A
+ * choice changes only how work is mapped to threads: results are
+ * bit-identical.
  * pf_set_tiles fails with PF_EUNSUPPORTED for a variant that does not exist
  * at this plan's lengths; pf_get_tiles returns the plan's choices and
  * whether the stage pass is fused. */

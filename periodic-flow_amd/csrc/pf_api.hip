@@ -618,17 +618,48 @@ StageArgs stage_args(const pf_plan* p, void* uhat, int stage, const void* noise,
   return a;
 }
 
+// Stokes flow with implicit PC and no forcing: the predictor's result is
+// never read (the corrector uses u0, L u0 and the noise only), so a step is
+// the corrector alone.
+bool corrector_only(const pf_plan* p) {
+  return !p->d.nonlinear && p->d.forcing == PF_FORCING_NONE && p->d.integrator == PF_IMPLICIT_PC && p->fuse_stage;
+}
+
+// Such steps with the on-device noise stream (or none) run many at a time
+// with the state in registers (stokes_steps_kernel), unless the stage pass's
+// launch variant 1 asks for one launch per step.  Both give the same state
+// bit for bit.
+bool resident_steps(const pf_plan* p, const void* noise) {
+  return corrector_only(p) && noise == nullptr && p->tsel[PF_PASS_STAGE] == 0;
+}
+
+constexpr int kResidentChunk = 200;  // steps per register-resident launch
+
+// n_steps corrector-only steps with the state in registers; acc (nullable)
+// += sum over members and components of |u_hat|^2 after every step.
+template <typename T> int resident_impl(pf_plan* p, void* uhat, long long n_steps, double* acc, hipStream_t s) {
+  const bool noise_on = p->d.noise_mag != 0.0;
+  StageArgs a = stage_args(p, uhat, 1, nullptr, noise_on);
+  a.save_u0 = 2;
+  a.single_draw = 1;
+  for (long long done = 0; done < n_steps;) {
+    const int k = (int)std::min<long long>(kResidentChunk, n_steps - done);
+    { PassTimer t(p, PF_PASS_STAGE, s); PF_TRY(launch_stokes_steps<T>(p->dim, a, k, acc, s)); }
+    PF_TRY(launch_member_step(p->msteps, p->nact, s, k));
+    done += k;
+    p->step += k;
+  }
+  return 0;
+}
+
 template <typename T> int step_impl(pf_plan* p, void* uhat, long long n_steps, const void* noise, hipStream_t s) {
+  if (resident_steps(p, noise)) return resident_impl<T>(p, uhat, n_steps, nullptr, s);
   const int ns = n_stages(p->d.integrator);
   const bool noise_on = p->d.noise_mag != 0.0 && uses_noise(p->d.integrator);
   const long long noise_block = (long long)p->nact * 2 * p->nc * p->nk;  // complex per step
   const bool fused = p->fuse_stage &&
                      tile_cols<T>(p->M[0], p->dim == 3 ? K_FWD_STAGE3 : K_FWD_STAGE2, p->tsel[PF_PASS_FWD0_STAGE]) > 0;
-  // Stokes flow with implicit PC and no forcing: the predictor's result is
-  // never read (the corrector uses u0, L u0 and the noise only), so the step
-  // is the corrector alone.
-  const int first = (!p->d.nonlinear && p->d.forcing == PF_FORCING_NONE && p->d.integrator == PF_IMPLICIT_PC &&
-                     p->fuse_stage) ? 1 : 0;
+  const int first = corrector_only(p) ? 1 : 0;
   for (long long st = 0; st < n_steps; ++st) {
     const void* nz = noise ? (const char*)noise + (size_t)(st * noise_block) * p->csz : nullptr;
     for (int stage = first; stage < ns; ++stage) {
@@ -697,6 +728,8 @@ template <typename T> bool variant_ok(const pf_plan* p, int pass, int v) {
     case PF_PASS_FWD1: return d3 && tile_cols<T>(p->M[1], K_FWD, v) > 0;
     case PF_PASS_FWD0: return tile_cols<T>(p->M[0], K_FWD, v) > 0;
     case PF_PASS_FWD0_STAGE: return tile_cols<T>(p->M[0], d3 ? K_FWD_STAGE3 : K_FWD_STAGE2, v) > 0;
+    // corrector-only Stokes steps: 0 register-resident, 1 one launch per step
+    case PF_PASS_STAGE: return v == 1 && corrector_only(p);
     default: return false;
   }
 }
@@ -1083,6 +1116,22 @@ int pf_step(pf_plan* p, void* uhat, int64_t n_steps, const void* noise, void* st
   if (p->nact == 0 || n_steps == 0) return PF_OK;
   return p->fp64 ? step_impl<double>(p, uhat, n_steps, noise, (hipStream_t)stream)
                  : step_impl<float>(p, uhat, n_steps, noise, (hipStream_t)stream);
+}
+
+int pf_step_accumulate_sk(pf_plan* p, void* uhat, int64_t n_steps, double* acc, void* stream) {
+  if (!p || !uhat || !acc || n_steps < 0) return fail(PF_EINVAL, "invalid argument");
+  if (p->nact == 0 || n_steps == 0) return PF_OK;
+  hipStream_t s = (hipStream_t)stream;
+  if (resident_steps(p, nullptr))
+    return p->fp64 ? resident_impl<double>(p, uhat, n_steps, acc, s) : resident_impl<float>(p, uhat, n_steps, acc, s);
+  for (int64_t i = 0; i < n_steps; ++i) {
+    int rc = p->fp64 ? step_impl<double>(p, uhat, 1, nullptr, s) : step_impl<float>(p, uhat, 1, nullptr, s);
+    if (rc) return rc;
+    rc = p->fp64 ? launch_sk<double>(p->dim, uhat, acc, p->nk, p->nact, s)
+                 : launch_sk<float>(p->dim, uhat, acc, p->nk, p->nact, s);
+    if (rc) return launch_fail(rc, "accumulate_sk");
+  }
+  return PF_OK;
 }
 
 int pf_compute_rhs(pf_plan* p, void* uhat, double a, const void* noise, void* rhs_nonlinear, void* rhs_linear,

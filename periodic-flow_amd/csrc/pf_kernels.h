@@ -469,77 +469,22 @@ template <typename T, int D> __device__ __forceinline__ void project(cplx<T>* v,
   for (int i = 0; i < D; ++i) v[i] = sub<T>(v[i], scl<T>(p, m.k[i]));
 }
 
-// One mode of one member: assemble the stage's terms and update u_hat.
-// nlin: the unprojected nonlinear term of this mode when the caller already
-// holds it (fused forward pass), else it is read from a.rhs_nl.
-// upre / uzpre: u_hat and u0 of this mode already fetched (stage_fetch).
-// mpre: the mode's wavenumbers when the caller knows its grid indices.
+// The nonlinear, noise and forcing terms of one mode of member b at RNG
+// step stp (space_solver.py:240-300): nl projected (math.py:72-100) and
+// Nyquist-masked, nz the projected noise scaled by noise_mag, fr the band
+// forcing.  nlin: the unprojected nonlinear term when the caller holds it,
+// else it is read from a.rhs_nl (or zero without a nonlinear term).
 template <typename T, int D>
-__device__ __forceinline__ void stage_point(const StageArgs& a, long long idx, int b, const cplx<T>* nlin,
-                                            const cplx<T>* upre = nullptr, const cplx<T>* uzpre = nullptr,
-                                            const Mode<T, D>* mpre = nullptr) {
-  const Mode<T, D> m = mpre ? *mpre : decode<T, D>(a, idx);
+__device__ __forceinline__ void stage_terms(const StageArgs& a, long long idx, int b, const cplx<T>* nlin,
+                                            const cplx<T>* u, const Mode<T, D>& m, long long stp, cplx<T>* nl,
+                                            cplx<T>* nz, cplx<T>* fr) {
   const long long mb = (long long)b * D * a.nk;
-  cplx<T>* uh = (cplx<T>*)a.uhat + mb + idx;
-  cplx<T>* u0p = (cplx<T>*)a.u0 + mb + idx;
-
-  cplx<T> u[D], uz[D], nl[D], nz[D], fr[D];
 #pragma unroll
   for (int i = 0; i < D; ++i) {
-    u[i] = upre ? upre[i] : uh[i * a.nk];
-    uz[i] = (a.save_u0 || a.mode == 1) ? u[i] : (uzpre ? uzpre[i] : u0p[i * a.nk]);
     nl[i] = mk<T>(0, 0);
     nz[i] = mk<T>(0, 0);
     fr[i] = mk<T>(0, 0);
   }
-  if (a.save_u0 == 1 && a.mode == 0) {
-#pragma unroll
-    for (int i = 0; i < D; ++i) u0p[i * a.nk] = u[i];
-  }
-  const T Lk = a.lin_op ? ((const T*)a.lin_op)[idx] : (T)(-a.nu) * m.k2;
-
-  if (a.mode == 2) {
-    // given terms: the caller's arrays, exactly as compute_rhs left them
-    const cplx<T>* gn = (const cplx<T>*)a.t_nl + mb + idx;
-    cplx<T>* gl = (cplx<T>*)a.t_lin + mb + idx;
-    const cplx<T>* gf = (const cplx<T>*)a.t_force + mb + idx;
-    const cplx<T>* gz = (const cplx<T>*)a.t_noise + mb + idx;
-    const T dt = (T)a.dt, ds = (T)a.dt_sqrt, ca = (T)a.ca, cb = (T)a.cb;
-#pragma unroll
-    for (int i = 0; i < D; ++i) {
-      const cplx<T> n_ = gn[i * a.nk], f_ = gf[i * a.nk], z_ = gz[i * a.nk];
-      cplx<T> l_ = gl[i * a.nk];
-      cplx<T> r;
-      switch (a.integ) {
-        case EXPLICIT_PC:
-          r = add<T>(add<T>(uz[i], scl<T>(add<T>(add<T>(n_, l_), f_), ca * dt)), scl<T>(z_, ds));
-          break;
-        case IMPLICIT_PC: {
-          if (a.last_stage) { l_ = scl<T>(uz[i], Lk); gl[i * a.nk] = l_; }
-          const T inv = (T)1 / ((T)1 - dt / (T)2 * Lk);
-          cplx<T> s_ = add<T>(uz[i], scl<T>(add<T>(n_, f_), ca * dt));
-          s_ = add<T>(s_, scl<T>(l_, cb * dt));
-          s_ = add<T>(s_, scl<T>(z_, ds));
-          r = scl<T>(s_, inv);
-          break;
-        }
-        case RK2:
-          r = add<T>(scl<T>(uz[i], ca), scl<T>(add<T>(u[i], scl<T>(add<T>(add<T>(n_, l_), f_), dt)), cb));
-          break;
-        case RK3: {
-          const cplx<T> in = add<T>(add<T>(u[i], scl<T>(add<T>(add<T>(n_, l_), f_), dt)), scl<T>(z_, ds));
-          r = add<T>(scl<T>(uz[i], ca), scl<T>(in, cb));
-          break;
-        }
-        default:
-          r = add<T>(uz[i], scl<T>(add<T>(add<T>(n_, l_), f_), ca * dt));
-          break;
-      }
-      uh[i * a.nk] = r;
-    }
-    return;
-  }
-
   // nonlinear term: Leray projection (math.py:72-100) and optional Nyquist mask
   if (nlin || a.rhs_nl) {
     if (nlin) {
@@ -565,7 +510,6 @@ __device__ __forceinline__ void stage_point(const StageArgs& a, long long idx, i
     const int nc = a.noise_type == 0 ? D * D : D;
     const cplx<T>* raw = a.noise ? (const cplx<T>*)a.noise + (long long)b * 2 * nc * a.nk + idx : nullptr;
     cplx<T> w[D * D];
-    const long long stp = a.msteps ? a.msteps[b] : a.step;
     // entry e of the on-device stream: w1 + a w2, or sqrt(1 + a^2) w with
     // one draw when the step has a single noise stage
     const T sfac = fac * (T)sqrt(1.0 + a.noise_a * a.noise_a);
@@ -634,22 +578,14 @@ __device__ __forceinline__ void stage_point(const StageArgs& a, long long idx, i
 #pragma unroll
     for (int i = 0; i < D; ++i) fr[i] = scl<T>(u[i], fc);
   }
+}
 
-  if (a.mode == 1) {
-    cplx<T>* on = (cplx<T>*)a.t_nl + mb + idx;
-    cplx<T>* ol = (cplx<T>*)a.t_lin + mb + idx;
-    cplx<T>* of = (cplx<T>*)a.t_force + mb + idx;
-    cplx<T>* oz = (cplx<T>*)a.t_noise + mb + idx;
-#pragma unroll
-    for (int i = 0; i < D; ++i) {
-      on[i * a.nk] = nl[i];
-      ol[i * a.nk] = scl<T>(u[i], Lk);
-      of[i * a.nk] = fr[i];
-      oz[i * a.nk] = nz[i];
-    }
-    return;
-  }
-
+// The integrator's update of one mode (time_integrator.py:51-318) from its
+// terms: u the stage input, uz the step's start, Lk the linear operator.
+template <typename T, int D>
+__device__ __forceinline__ void stage_combine(const StageArgs& a, const cplx<T>* u, const cplx<T>* uz,
+                                              const cplx<T>* nl, const cplx<T>* nz, const cplx<T>* fr, T Lk,
+                                              cplx<T>* out) {
   const T dt = (T)a.dt, ds = (T)a.dt_sqrt, ca = (T)a.ca, cb = (T)a.cb;
 #pragma unroll
   for (int i = 0; i < D; ++i) {
@@ -686,8 +622,100 @@ __device__ __forceinline__ void stage_point(const StageArgs& a, long long idx, i
         break;
       }
     }
-    uh[i * a.nk] = r;
+    out[i] = r;
   }
+}
+
+// One mode of one member: assemble the stage's terms and update u_hat.
+// nlin: the unprojected nonlinear term of this mode when the caller already
+// holds it (fused forward pass), else it is read from a.rhs_nl.
+// upre / uzpre: u_hat and u0 of this mode already fetched (stage_fetch).
+// mpre: the mode's wavenumbers when the caller knows its grid indices.
+template <typename T, int D>
+__device__ __forceinline__ void stage_point(const StageArgs& a, long long idx, int b, const cplx<T>* nlin,
+                                            const cplx<T>* upre = nullptr, const cplx<T>* uzpre = nullptr,
+                                            const Mode<T, D>* mpre = nullptr) {
+  const Mode<T, D> m = mpre ? *mpre : decode<T, D>(a, idx);
+  const long long mb = (long long)b * D * a.nk;
+  cplx<T>* uh = (cplx<T>*)a.uhat + mb + idx;
+  cplx<T>* u0p = (cplx<T>*)a.u0 + mb + idx;
+
+  cplx<T> u[D], uz[D], nl[D], nz[D], fr[D];
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    u[i] = upre ? upre[i] : uh[i * a.nk];
+    uz[i] = (a.save_u0 || a.mode == 1) ? u[i] : (uzpre ? uzpre[i] : u0p[i * a.nk]);
+  }
+  if (a.save_u0 == 1 && a.mode == 0) {
+#pragma unroll
+    for (int i = 0; i < D; ++i) u0p[i * a.nk] = u[i];
+  }
+  const T Lk = a.lin_op ? ((const T*)a.lin_op)[idx] : (T)(-a.nu) * m.k2;
+
+  if (a.mode == 2) {
+    // given terms: the caller's arrays, exactly as compute_rhs left them
+    const cplx<T>* gn = (const cplx<T>*)a.t_nl + mb + idx;
+    cplx<T>* gl = (cplx<T>*)a.t_lin + mb + idx;
+    const cplx<T>* gf = (const cplx<T>*)a.t_force + mb + idx;
+    const cplx<T>* gz = (const cplx<T>*)a.t_noise + mb + idx;
+    const T dt = (T)a.dt, ds = (T)a.dt_sqrt, ca = (T)a.ca, cb = (T)a.cb;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      const cplx<T> n_ = gn[i * a.nk], f_ = gf[i * a.nk], z_ = gz[i * a.nk];
+      cplx<T> l_ = gl[i * a.nk];
+      cplx<T> r;
+      switch (a.integ) {
+        case EXPLICIT_PC:
+          r = add<T>(add<T>(uz[i], scl<T>(add<T>(add<T>(n_, l_), f_), ca * dt)), scl<T>(z_, ds));
+          break;
+        case IMPLICIT_PC: {
+          if (a.last_stage) { l_ = scl<T>(uz[i], Lk); gl[i * a.nk] = l_; }
+          const T inv = (T)1 / ((T)1 - dt / (T)2 * Lk);
+          cplx<T> s_ = add<T>(uz[i], scl<T>(add<T>(n_, f_), ca * dt));
+          s_ = add<T>(s_, scl<T>(l_, cb * dt));
+          s_ = add<T>(s_, scl<T>(z_, ds));
+          r = scl<T>(s_, inv);
+          break;
+        }
+        case RK2:
+          r = add<T>(scl<T>(uz[i], ca), scl<T>(add<T>(u[i], scl<T>(add<T>(add<T>(n_, l_), f_), dt)), cb));
+          break;
+        case RK3: {
+          const cplx<T> in = add<T>(add<T>(u[i], scl<T>(add<T>(add<T>(n_, l_), f_), dt)), scl<T>(z_, ds));
+          r = add<T>(scl<T>(uz[i], ca), scl<T>(in, cb));
+          break;
+        }
+        default:
+          r = add<T>(uz[i], scl<T>(add<T>(add<T>(n_, l_), f_), ca * dt));
+          break;
+      }
+      uh[i * a.nk] = r;
+    }
+    return;
+  }
+
+  const long long stp = a.msteps ? a.msteps[b] : a.step;
+  stage_terms<T, D>(a, idx, b, nlin, u, m, stp, nl, nz, fr);
+
+  if (a.mode == 1) {
+    cplx<T>* on = (cplx<T>*)a.t_nl + mb + idx;
+    cplx<T>* ol = (cplx<T>*)a.t_lin + mb + idx;
+    cplx<T>* of = (cplx<T>*)a.t_force + mb + idx;
+    cplx<T>* oz = (cplx<T>*)a.t_noise + mb + idx;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      on[i * a.nk] = nl[i];
+      ol[i * a.nk] = scl<T>(u[i], Lk);
+      of[i * a.nk] = fr[i];
+      oz[i * a.nk] = nz[i];
+    }
+    return;
+  }
+
+  cplx<T> r[D];
+  stage_combine<T, D>(a, u, uz, nl, nz, fr, Lk, r);
+#pragma unroll
+  for (int i = 0; i < D; ++i) uh[i * a.nk] = r[i];
 }
 
 // Loads of stage_point hoisted so that several modes' loads are in flight.
@@ -721,6 +749,46 @@ __global__ void __launch_bounds__(kThreads) stage_kernel(StageArgs a) {
     if (a.mode == 2) stage_point<T, D>(a, idx, b, nullptr);
     else stage_point<T, D>(a, idx, b, nullptr, uu[q], zz[q]);
   }
+}
+
+// Stokes flow stepped with the state in registers (time_integrator.py:111-129
+// run with is_nonlinear = False, space_solver.py:227-265): the step of
+// implicit PC whose predictor is dead, u <- (u + dt/2 L u + sqrt(dt/2) n) /
+// (1 - dt/2 L), for nsteps steps of members [b0, b0 + MB) of one mode, with
+// on-device noise keyed by each member's step counter.  Each step is the
+// arithmetic of stage_point for that step (bitwise the same state).  acc
+// (nullable) accumulates sum over the members, components and every step of
+// |u_hat|^2 in FP64 in registers, added to acc[idx] once at the end.
+template <typename T, int D, int MB>
+__global__ void __launch_bounds__(kThreads) stokes_steps_kernel(StageArgs a, int nsteps, double* acc) {
+  const long long idx = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (idx >= a.nk) return;
+  const Mode<T, D> m = decode<T, D>(a, idx);
+  const T Lk = a.lin_op ? ((const T*)a.lin_op)[idx] : (T)(-a.nu) * m.k2;
+  const int b0 = blockIdx.y * MB;
+  double s = 0.0;
+  for (int bb = 0; bb < MB; ++bb) {
+    const int b = b0 + bb;
+    if (b >= a.n_members) break;
+    cplx<T>* uh = (cplx<T>*)a.uhat + (long long)b * D * a.nk + idx;
+    cplx<T> u[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) u[i] = uh[i * a.nk];
+    const long long st0 = a.msteps[b];
+    for (int t = 0; t < nsteps; ++t) {
+      cplx<T> nl[D], nz[D], fr[D], r[D];
+      stage_terms<T, D>(a, idx, b, nullptr, u, m, st0 + t, nl, nz, fr);
+      stage_combine<T, D>(a, u, u, nl, nz, fr, Lk, r);
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+        u[i] = r[i];
+        if (acc) s += (double)r[i].x * (double)r[i].x + (double)r[i].y * (double)r[i].y;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < D; ++i) uh[i * a.nk] = u[i];
+  }
+  if (acc) atomicAdd(acc + idx, s);
 }
 
 // ------------------------------------------------------------- reductions (FP64)

@@ -98,6 +98,7 @@ int launch_fill(double* p, long long n, double v, hipStream_t s);
 template <typename T>
 int launch_draw(void* out, const unsigned long long* seeds, const long long* msteps, long long nk, int nc, int nmem,
                 hipStream_t s);
-int launch_member_step(long long* msteps, int n, hipStream_t s);
+int launch_member_step(long long* msteps, int n, hipStream_t s, long long by = 1);
+template <typename T> int launch_stokes_steps(int dim, const StageArgs& a, int nsteps, double* acc, hipStream_t s);
 
 }  // namespace pf

@@ -3,10 +3,10 @@
 
 namespace pf {
 
-// one step done: advance the RNG step counter of every active member
-__global__ void member_step_kernel(long long* msteps, int n) {
+// steps done: advance the RNG step counter of every active member
+__global__ void member_step_kernel(long long* msteps, int n, long long by) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) msteps[i] += 1;
+  if (i < n) msteps[i] += by;
 }
 
 __global__ void fill_kernel(double* p, long long n, double v) {
@@ -117,9 +117,29 @@ int launch_draw(void* out, const unsigned long long* seeds, const long long* mst
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
-int launch_member_step(long long* msteps, int n, hipStream_t s) {
+int launch_member_step(long long* msteps, int n, hipStream_t s, long long by) {
   if (n <= 0) return 0;
-  hipLaunchKernelGGL(member_step_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, msteps, n);
+  hipLaunchKernelGGL(member_step_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, msteps, n, by);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}
+
+// Members per thread of the register-resident Stokes kernel: enough blocks
+// along the members that the launch fills the GPU with about 8 waves per
+// SIMD (256 CUs x 32 waves), and no more (each thread keeps its mode's
+// wavenumbers and linear operator for all of its members).
+template <typename T> int launch_stokes_steps(int dim, const StageArgs& a, int nsteps, double* acc, hipStream_t s) {
+  const long long blocks_x = (a.nk + kThreads - 1) / kThreads;
+  const long long want = 256LL * 32 * 64 / kThreads;  // blocks for 32 waves on each of 256 CUs
+  long long mb = blocks_x * a.n_members / want;
+  const int MBv = mb >= 8 ? 8 : (mb >= 4 ? 4 : (mb >= 2 ? 2 : 1));
+  const dim3 grid((unsigned)blocks_x, (unsigned)((a.n_members + MBv - 1) / MBv));
+#define PF_SK(D, M) hipLaunchKernelGGL((stokes_steps_kernel<T, D, M>), grid, dim3(kThreads), 0, s, a, nsteps, acc)
+  if (dim == 3) {
+    switch (MBv) { case 8: PF_SK(3, 8); break; case 4: PF_SK(3, 4); break; case 2: PF_SK(3, 2); break; default: PF_SK(3, 1); }
+  } else {
+    switch (MBv) { case 8: PF_SK(2, 8); break; case 4: PF_SK(2, 4); break; case 2: PF_SK(2, 2); break; default: PF_SK(2, 1); }
+  }
+#undef PF_SK
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
@@ -130,7 +150,8 @@ int launch_member_step(long long* msteps, int n, hipStream_t s) {
   template int launch_spectrum<T>(int, const DiagArgs&, int, int, hipStream_t);                       \
   template int launch_band<T>(int, const BandArgs&, int, hipStream_t);                                \
   template int launch_sk<T>(int, const void*, double*, long long, int, hipStream_t);                  \
-  template int launch_draw<T>(void*, const unsigned long long*, const long long*, long long, int, int, hipStream_t);
+  template int launch_draw<T>(void*, const unsigned long long*, const long long*, long long, int, int, hipStream_t); \
+  template int launch_stokes_steps<T>(int, const StageArgs&, int, double*, hipStream_t);
 PF_INST(float)
 PF_INST(double)
 #undef PF_INST

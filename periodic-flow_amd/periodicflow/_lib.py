@@ -77,6 +77,7 @@ _SIGS = {
     "pf_rhs_nonlinear": (C.c_int, [_P, _P, _P, _P]),
     "pf_rhs_nonlinear_noalloc": (C.c_int, [C.POINTER(PfDesc), _P, _P, _P]),
     "pf_step": (C.c_int, [_P, _P, C.c_int64, _P, _P]),
+    "pf_step_accumulate_sk": (C.c_int, [_P, _P, C.c_int64, _P, _P]),
     "pf_compute_rhs": (C.c_int, [_P, _P, C.c_double, _P, _P, _P, _P, _P, _P]),
     "pf_stage_update": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P, _P, _P, _P]),
     "pf_draw_noise": (C.c_int, [_P, _P, _P]),

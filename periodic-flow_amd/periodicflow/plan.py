@@ -264,8 +264,10 @@ class Plan:
 
     def set_tiles(self, tiles):
         """Launch variant per pass (dict by pass name, unnamed passes 0): 0 the
-        default tile width or row-pair count, 1 half of it, 2 twice it.  Results
-        are bit-identical across variants."""
+        default tile width or row-pair count, 1 half of it, 2 twice it; for
+        Stokes flow with implicit PC, "stage": 1 steps one launch per step
+        instead of many steps with the state in registers.  Results are
+        bit-identical across variants."""
         n = len(_lib.PASS_NAMES)
         unknown = set(tiles) - set(_lib.PASS_NAMES)
         if unknown:
@@ -304,6 +306,13 @@ class Plan:
         cnt = (C.c_int64 * n)()
         call("pf_profile_read", self._h, ms, cnt, n)
         return {nm: (float(ms[i]), int(cnt[i])) for i, nm in enumerate(_lib.PASS_NAMES)}
+
+    def step_accumulate_sk(self, u_hat, n_steps, acc):
+        """Advance n_steps with acc += sum over members and components of
+        |u_hat|^2 after every step (FP64 acc of shape shape_fourier)."""
+        self._check(u_hat, self.state_shape(), self.complex_dtype, "u_hat")
+        self._check(acc, self.shape_fourier, torch.float64, "acc")
+        call("pf_step_accumulate_sk", self._h, _ptr(u_hat), int(n_steps), _ptr(acc), _stream())
 
     def accumulate_sk(self, u_hat, acc):
         self._check(u_hat, self.state_shape(), self.complex_dtype, "u_hat")

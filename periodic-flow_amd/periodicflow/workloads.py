@@ -81,10 +81,10 @@ def stokes512(members=256, steps=2000, burn=200, n=512, nu=1.0, kT=1e-3, dt=0.05
     t_burn = _timed(lambda: p.step(u, burn))
 
     def run():
-        for _ in range(steps - burn):
-            p.step(u, 1)
-            if sk:
-                p.accumulate_sk(u, acc)
+        if sk:
+            p.step_accumulate_sk(u, steps - burn, acc)
+        else:
+            p.step(u, steps - burn)
 
     t = _timed(run)
     S = acc.cpu().numpy() / ((steps - burn) * members)

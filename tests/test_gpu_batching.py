@@ -136,9 +136,7 @@ def test_structure_factor_flat(mods):
     p.step(u, 200)
     acc = torch.zeros(p.shape_fourier, dtype=torch.float64, device="cuda")
     nsamp = 400
-    for _ in range(nsamp):
-        p.step(u, 1)
-        p.accumulate_sk(u, acc)
+    p.step_accumulate_sk(u, nsamp, acc)
     S = acc.cpu().numpy() / (nsamp * B)
     theory = nm**2 * 2 / (2 * nu * n * n)
     k0 = np.fft.fftfreq(n, 1.0 / n)[:, None]
@@ -153,3 +151,43 @@ def test_structure_factor_flat(mods):
         m = shells == s
         if m.sum() >= 50:
             assert abs(ratio[m].mean() - 1) < 0.05, s
+
+
+# (256, 256) with 140 members runs 8 members per thread with a partial last group
+@pytest.mark.parametrize("N,ntype,prec,B", [((256, 256), "thermal", "fp32", 140), ((64, 64), "thermal", "fp64", 3),
+                                           ((16, 16, 16), "correlated", "fp32", 5),
+                                           ((12, 16, 8), "thermal", "fp64", 2)])
+def test_register_resident_stokes_equals_per_step(mods, N, ntype, prec, B):
+    """Implicit-PC Stokes steps with the state in registers (default) give the
+    state of one launch per step (stage variant 1) bit for bit, and the FP64
+    structure factor summed in registers matches per-step accumulation."""
+    _, plan, _ = mods
+    _say("resident stokes", N, ntype, prec, B)
+    kw = dict(precision=prec, integrator="implicit_pc", nonlinear=False, viscosity=0.7, noise_type=ntype,
+              noise_mag=0.4, batch=B, seeds=[50 + b for b in range(B)])
+    pr = plan.Plan(N, 0.03, **kw)
+    ps = plan.Plan(N, 0.03, **kw)
+    ps.set_tiles({"stage": 1})
+    if ntype == "correlated":
+        C = torch.rand(pr.shape_fourier, dtype=pr.real_dtype, device="cuda").to(pr.complex_dtype)
+        pr.set_correlation(C)
+        ps.set_correlation(C)
+    rs = np.random.RandomState(3)
+    u0 = rs.standard_normal(pr.state_shape()) + 1j * rs.standard_normal(pr.state_shape())
+    ur = torch.from_numpy(u0).to("cuda", pr.complex_dtype)
+    us = ur.clone()
+    pr.step(ur, 7)
+    ps.step(us, 7)
+    assert torch.equal(ur, us)
+    acc_r = torch.zeros(pr.shape_fourier, dtype=torch.float64, device="cuda")
+    acc_s = torch.zeros_like(acc_r)
+    pr.step_accumulate_sk(ur, 230, acc_r)  # crosses the 200-step launch chunk
+    for _ in range(230):
+        ps.step(us, 1)
+        ps.accumulate_sk(us, acc_s)
+    assert torch.equal(ur, us)
+    assert pr.step_counter == ps.step_counter == 237
+    assert pr.member_steps() == ps.member_steps() == [237] * B
+    rel = float(torch.max(torch.abs(acc_r - acc_s) / acc_s.clamp_min(1e-300)))
+    _say("   S(k) accumulation max rel diff", rel)
+    assert rel < 1e-12
