@@ -13,8 +13,16 @@ the 8 TB/s HBM peak using the algorithmic bytes of periodicflow.perfmodel
 and the pass's mean duration from HIP events recorded around each of its
 launches inside the timed region.  ``cpu_baseline`` times the FP64 NumPy
 oracle (scipy.fft over all host cores) on a bounded sample of the same
-workload.  The FP32 result is compared with the FP64 GPU path after the same
-steps (``fp32_rel_l2_vs_fp64``).
+workload, and ``accuracy_vs_oracle`` compares the GPU's FP64 and FP32 states
+with the oracle's after those steps from the same start.  The FP32 result is
+also compared with the FP64 GPU path after all the timed steps
+(``fp32_rel_l2_vs_fp64``).
+
+Beside the headline: ``batched_vs_plain`` (thermal-noise Stokes 512^2 x 256
+realisations; forced turbulence 32 x 64^3; forced turbulence 16 x 128^3 +
+16 x 96^3 at its own sizes; each against the same members one at a time)
+and ``serving`` (a bounded request stream: continuous batching against one
+at a time, throughput and p50/p99 latency).
 """
 import argparse
 import json
@@ -49,17 +57,32 @@ def _tgv_state(plan, torch):
     return uh
 
 
-def cpu_baseline(steps):
+def cpu_baseline(steps, torch=None):
+    """The oracle timed on the host's cores; with ``torch`` also the GPU's
+    FP64 and FP32 states after the same steps from the same start, against
+    the oracle's."""
     from oracle import pf_oracle as po
 
     cores = len(os.sched_getaffinity(0))
     po.set_fft_workers(cores)
     s = po.OracleSolver([N] * 3, DT, time_integrator="rk4", viscosity=NU, dealias="2/3")
     s.initialize(po.taylor_green_3d(s.g))
+    start = s.u_hat.copy()
     t0 = time.perf_counter()
     s.solve(steps)
     dt = time.perf_counter() - t0
-    return {
+    acc = None
+    if torch is not None:
+        from periodicflow.plan import Plan
+
+        acc = {"steps": steps}
+        for prec in ("fp64", "fp32"):
+            p = Plan([N] * 3, DT, precision=prec, integrator="rk4", viscosity=NU, dealias="2/3")
+            uh = torch.from_numpy(start[None].copy()).to("cuda", p.complex_dtype)
+            p.step(uh, steps)
+            acc[f"{prec}_rel_l2_vs_oracle"] = po.rel_l2(uh.cpu().numpy()[0], s.u_hat)
+            del p, uh
+    return acc, {
         "value": N**3 * steps / dt / 1e6,
         "unit": "Mpoint-steps/s",
         "cores": cores,
@@ -78,7 +101,7 @@ def _rate(torch, fn, points):
     return points / (time.perf_counter() - t0) / 1e6
 
 
-def batched_vs_plain(torch, members=256, steps=20, plain_members=32):
+def batched_vs_plain(torch, members=256, steps=100, plain_members=32):
     """Batched FP32 ensemble stepping vs the plain path (the same kernels at
     batch 1, one solve after another), on the 2D thermal-noise Stokes
     ensemble (512^2, implicit PC, dt = 0.05, nu = 1, kT = 1e-3, seeds
@@ -144,7 +167,63 @@ def batched_vs_plain(torch, members=256, steps=20, plain_members=32):
     rp = _rate(torch, plain3, n3**3 * 8 * st3)
     out["forced3d_64_rk3"] = {"members": m3, "steps": st3, "batched": round(rb, 1), "plain": round(rp, 1),
                               "plain_members_timed": 8, "speedup": round(rb / rp, 2)}
+    del pb, ub, pp, up
+    out["forced3d_16x128_16x96_rk3"] = forced_batched_vs_plain(torch)
     return out
+
+
+def forced_batched_vs_plain(torch, steps=5):
+    """The forced-turbulence ensemble at its sizes: 16 members at 128^3 and
+    16 at 96^3 (3/2 rule, RK3, correlated noise, constant-power forcing,
+    FP32), launch plans autotuned, against one member of each size alone
+    (batch 1, also autotuned) timed for the same steps and counted 16 times."""
+    from periodicflow import workloads as W
+    from periodicflow.ensemble import Ensemble, random_solenoidal
+
+    members = W.forced_members()
+
+    def build(ms):
+        e = Ensemble(ms, precision="fp32", correlation=W.forced_correlation)
+        for i, (spec, seed) in enumerate(ms):
+            e.set_fourier(i, random_solenoidal(e.plan_of(i), seed, energy=0.5, kp=4.0))
+        e.autotune()
+        e.step(1)
+        return e
+
+    ens = build(members)
+    pts = ens.points_per_step() * steps
+    rb = _rate(torch, lambda: ens.step(steps), pts)
+    del ens
+    t_plain = 0.0
+    for m in (members[0], members[16]):
+        e1 = build([m])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e1.step(steps)
+        torch.cuda.synchronize()
+        t_plain += 16 * (time.perf_counter() - t0)
+        del e1
+    rp = pts / t_plain / 1e6
+    return {"members": len(members), "steps": steps, "batched": round(rb, 1), "plain": round(rp, 1),
+            "speedup": round(rb / rp, 2)}
+
+
+def serving_bench(torch, count=40, max_steps=300):
+    """A bounded run of the request stream (Poisson arrivals at 20/s, seed
+    42; grids 128^2..1024^2 and 64^3..256^3; Taylor-Green, thermal Stokes or
+    forced turbulence; steps log-uniform in [100, max_steps]; FP32):
+    continuous batching against the same requests one at a time."""
+    from periodicflow import workloads as W
+    from periodicflow.serving import Engine, GpuBackend, serve_one_at_a_time
+
+    reqs = W.serving_requests(count=count, max_steps=max_steps)
+    batched = Engine(GpuBackend("fp32"), hbm_budget=96 << 30, keep_results=False).run(reqs)
+    reqs = W.serving_requests(count=count, max_steps=max_steps)
+    serial = serve_one_at_a_time(GpuBackend("fp32"), reqs, hbm_budget=96 << 30, keep_results=False)[0]
+    keep = ("done", "makespan_s", "Mpoint_steps_per_s", "p50_latency_s", "p99_latency_s")
+    return {"requests": count, "max_steps": max_steps,
+            "batched": {k: batched[k] for k in keep}, "one_at_a_time": {k: serial[k] for k in keep},
+            "throughput_speedup": round(batched["Mpoint_steps_per_s"] / max(serial["Mpoint_steps_per_s"], 1e-9), 2)}
 
 
 def suite(torch, path):
@@ -178,6 +257,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-check", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch plans")
     ap.add_argument("--suite", default=None, help="also run the long configurations, results to this JSON file")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -325,11 +405,14 @@ def main():
     }
     if not args.no_batched:
         out["batched_vs_plain"] = batched_vs_plain(torch)
+    if not args.no_serving:
+        out["serving"] = serving_bench(torch)
     if args.suite:
         suite(torch, args.suite)
         out["suite"] = args.suite
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
+        acc, out["cpu_baseline"] = cpu_baseline(args.cpu_steps, torch)
+        out["accuracy_vs_oracle"] = acc
     print(json.dumps(out), flush=True)
 
 

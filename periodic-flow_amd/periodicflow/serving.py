@@ -8,11 +8,20 @@ together.  The engine repeatedly
 1. admits waiting requests in arrival order, into a free slot of their pool
    or a new pool, while the pools' device footprint stays within the HBM
    budget (idle pools are released to make room);
-2. advances every pool with active members by a chunk of steps, no longer
-   than the smallest remaining step count among them;
+2. advances pools by chunks of steps, never past the smallest remaining
+   step count among a pool's members.  Under the default policy ('srpt')
+   the pool whose shortest member has the least work left goes first, with a
+   chunk of about ``round_s`` seconds (from the backend's per-step cost
+   estimate); further pools, in the same order, join the round while the
+   estimated work of the round stays within ``overlap`` times the first
+   pool's.  Under 'all' every pool advances by about ``round_s`` seconds
+   each round.  The GPU backend gives each pool one of a few HIP streams, so
+   the launches of different pools in a round run side by side on the
+   device;
 3. retires members that finished (or went non-finite), moving the pool's
    last active member into the freed slot so active members stay in
-   slots [0, n).
+   slots [0, n).  Energies are read back only for pools with a finishing
+   member and, to catch non-finite members, every ``check_every`` rounds.
 
 A member's arithmetic and noise stream (keyed by its seed and its own step
 counter) do not depend on its neighbours or on when it joined, so each
@@ -69,9 +78,15 @@ class Pool:
 
 
 class GpuBackend:
-    """Pools as batched HIP plans; states are device tensors."""
+    """Pools as batched HIP plans; states are device tensors.
 
-    def __init__(self, precision="fp32", correlation=None):
+    Pools are dealt round-robin to ``n_streams`` HIP streams (default
+    priority; the box exposes 4 hardware queues per process), so the steps
+    of different pools overlap on the device.  Loads, moves and read-backs
+    run on the default stream; each round waits for them before stepping and
+    the engine synchronises the device at the end of every round."""
+
+    def __init__(self, precision="fp32", correlation=None, n_streams=4):
         import torch
 
         from .plan import Plan
@@ -80,6 +95,20 @@ class GpuBackend:
         self.Plan = Plan
         self.precision = precision
         self.correlation = correlation
+        self.streams = [torch.cuda.Stream() for _ in range(max(1, int(n_streams)))]
+        self._next = 0
+
+    def step_cost(self, spec, n):
+        """Estimated seconds of one step of n members: algorithmic bytes at
+        4 TB/s plus launch overhead for Navier-Stokes; register-resident
+        Stokes flow at 300 G point-steps/s."""
+        from .perfmodel import step_bytes
+
+        stages = {"rk3": 3, "rk4": 4}.get(spec.integrator, 2)
+        if spec.nonlinear:
+            b = step_bytes(spec.n, spec.dealias, self.precision, stages, batch=n)
+            return b / 4.0e12 + stages * 5 * 4e-6
+        return n * int(np.prod(spec.n)) / 3.0e11 + 5e-6
 
     def footprint(self, spec, capacity):
         """Device bytes of a pool: plan workspace + state (estimated before creation)."""
@@ -106,7 +135,9 @@ class GpuBackend:
             C = self.torch.as_tensor(np.asarray(self.correlation(spec)), device="cuda", dtype=p.complex_dtype)
             p.set_correlation(C.contiguous())
         p.active = 0
-        return {"plan": p, "state": p.new_state()}
+        st = self.streams[self._next % len(self.streams)]
+        self._next += 1
+        return {"plan": p, "state": p.new_state(), "stream": st}
 
     def load(self, h, slot, req):
         p = h["plan"]
@@ -123,7 +154,10 @@ class GpuBackend:
         h["plan"].active = n
 
     def step(self, h, k):
-        h["plan"].step(h["state"], k)
+        s = h["stream"]
+        s.wait_stream(self.torch.cuda.default_stream())
+        with self.torch.cuda.stream(s):
+            h["plan"].step(h["state"], k)
 
     def energies(self, h, n):
         if n == 0:
@@ -137,12 +171,20 @@ class GpuBackend:
         self.torch.cuda.synchronize()
 
     def release(self, h):
+        self.torch.cuda.synchronize()
         h.clear()
 
 
 class Engine:
-    def __init__(self, backend, hbm_budget=64 << 30, chunk=32, max_slots=64, keep_results=True,
-                 clock=time.perf_counter, policy="srpt"):
+    """``chunk``: most steps a pool advances per round; ``round_s``: target
+    device time of a pool's share of a round (with a backend step_cost);
+    ``overlap``: under 'srpt', how much estimated work the round may hold
+    relative to its first pool's; ``check_every``: rounds between energy
+    read-backs of pools with no finishing member; ``policy``: 'srpt' or
+    'all' (see the module docstring)."""
+
+    def __init__(self, backend, hbm_budget=64 << 30, chunk=200, max_slots=64, keep_results=True,
+                 clock=time.perf_counter, policy="srpt", round_s=0.05, check_every=8, overlap=1.25):
         self.be = backend
         self.budget = int(hbm_budget)
         self.chunk = int(chunk)
@@ -154,6 +196,10 @@ class Engine:
         self.finished: list = []
         self.t0 = None
         self.in_use = 0
+        self.rounds = 0
+        self.round_s = float(round_s)
+        self.check_every = max(1, int(check_every))
+        self.overlap = float(overlap)
         if policy not in ("srpt", "all"):
             raise ValueError("policy must be 'srpt' or 'all'")
         self.policy = policy
@@ -244,35 +290,55 @@ class Engine:
         self.finished.append(r)
         pool.last_used = r.t_done
 
-    def _order(self):
-        """Pools to advance this round.  'all': every pool with members.
-        'srpt': the pool whose shortest member has the least remaining work
-        (point-steps); members of a pool always advance together."""
+    def _remaining(self, pool):
+        return min((r.steps - r.steps_done) for r in pool.slots) * int(np.prod(pool.spec.n))
+
+    def _steps_this_round(self, pool):
+        k = self.chunk
+        cost = getattr(self.be, "step_cost", None)
+        if cost is not None:
+            k = min(k, max(1, int(self.round_s / max(cost(pool.spec, len(pool.slots)), 1e-9))))
+        return min(k, min(r.steps - r.steps_done for r in pool.slots))
+
+    def _round(self):
+        """[(pool, steps)] to advance this round."""
         act = [p for p in self.pools.values() if p.slots]
-        if self.policy == "all" or len(act) <= 1:
-            return act
-        def rem(p):
-            return min((r.steps - r.steps_done) for r in p.slots) * int(np.prod(p.spec.n))
-        return [min(act, key=rem)]
+        if self.policy == "all":
+            return [(p, self._steps_this_round(p)) for p in act]
+        act.sort(key=self._remaining)
+        if not act:
+            return []
+        head = act[0]
+        out = [(head, self._steps_this_round(head))]
+        cost = getattr(self.be, "step_cost", None)
+        if cost is None:
+            return out
+        used = out[0][1] * cost(head.spec, len(head.slots))
+        limit = self.overlap * used
+        for p in act[1:]:
+            k = self._steps_this_round(p)
+            c = k * cost(p.spec, len(p.slots))
+            if used + c <= limit:
+                out.append((p, k))
+                used += c
+        return out
 
     def _advance(self):
-        busy = False
-        for pool in self._order():
-            n = len(pool.slots)
-            if n == 0:
-                continue
-            busy = True
-            k = min(self.chunk, min(r.steps - r.steps_done for r in pool.slots))
+        rnd = self._round()
+        if not rnd:
+            return False
+        act = [p for p, _ in rnd]
+        for pool, k in rnd:
             if k > 0:
                 self.be.step(pool.handle, k)
                 for r in pool.slots:
                     r.steps_done += k
-        if not busy:
-            return False
         self.be.sync()
-        for pool in list(self.pools.values()):
+        self.rounds += 1
+        check = self.rounds % self.check_every == 0
+        for pool in act:
             n = len(pool.slots)
-            if n == 0:
+            if not (check or any(r.steps_done >= r.steps for r in pool.slots)):
                 continue
             e = self.be.energies(pool.handle, n)
             # retire from the back so slot indices stay valid
@@ -328,8 +394,9 @@ class Engine:
         }
 
 
-def serve_one_at_a_time(backend, requests, hbm_budget=64 << 30, chunk=1 << 20):
+def serve_one_at_a_time(backend, requests, hbm_budget=64 << 30, chunk=1 << 20, keep_results=True):
     """The plain baseline: each request alone (batch 1) in arrival order,
-    through the same engine; latency includes queueing."""
-    eng = Engine(backend, hbm_budget=hbm_budget, chunk=chunk, max_slots=1)
+    its whole step count in one call, through the same engine; latency
+    includes queueing."""
+    eng = Engine(backend, hbm_budget=hbm_budget, chunk=chunk, max_slots=1, round_s=1e9, keep_results=keep_results)
     return eng.run(requests, serial=True), eng.finished

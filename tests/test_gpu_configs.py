@@ -278,3 +278,112 @@ def test_anisotropic_box_rk3_thermal(mods, prec):
     err = po.rel_l2(_host(uh), np.stack(refs))
     _say(f"box {L} grid {N} RK3 thermal {prec}: rel L2 {err:.3e}")
     assert err < (FP64_TOL if prec == "fp64" else FP32_TOL)
+
+
+# ------------------------------------ on-device noise stream vs the oracle
+def _single_draw_raw(seed, step, nk, shape, d, ntype):
+    """The oracle's raw draws (w1, w2) that reproduce one step of the device's
+    single-draw stream (implicit PC Stokes, whose only live noise stage has
+    a = 1): entry e of the stream is word pair e % 2 of counter group e // 2
+    (po.device_noise_draws), scaled by sqrt(1 + a^2) = sqrt 2.  Thermal:
+    entries are the upper triangle of the symmetric tensor in row order; the
+    oracle symmetrises w1 (space_solver.py:15-42), so w1 holds the tensor
+    with sqrt 2 on the diagonal and w2 = 0.  Correlated: w1 = sqrt 2 g."""
+    ne = d * (d + 1) // 2 if ntype == "thermal" else d
+    groups = (ne + 1) // 2
+    dev = po.device_noise_draws([seed], [step], nk, groups)[0]  # (2, groups, nk)
+    g = [dev[e % 2, e // 2].reshape(shape) for e in range(ne)]
+    if ntype == "thermal":
+        w1 = np.zeros((d * d,) + shape, dtype=complex)
+        e = 0
+        for i in range(d):
+            for j in range(i, d):
+                if i == j:
+                    w1[i * d + i] = np.sqrt(2.0) * g[e]
+                else:
+                    w1[i * d + j] = w1[j * d + i] = g[e]
+                e += 1
+    else:
+        w1 = np.sqrt(2.0) * np.array(g)
+    return w1, np.zeros_like(w1)
+
+
+@pytest.mark.parametrize("N,ntype", [((512, 512), "thermal"), ((24, 16, 32), "thermal"), ((64, 64), "correlated")])
+def test_resident_stokes_device_noise_vs_oracle(mods, N, ntype):
+    """Register-resident Stokes steps with the on-device noise stream, FP64,
+    against the oracle fed the same draws: 6 steps, 2 members."""
+    plan_m, _, _ = mods
+    steps, B, nu, dt, nm = 6, 2, 1.0, 0.05, 0.3
+    seeds = [77, 78]
+    p = plan_m.Plan(N, dt, precision="fp64", integrator="implicit_pc", nonlinear=False, viscosity=nu,
+                    noise_type=ntype, noise_mag=nm, batch=B, seeds=seeds)
+    rs = np.random.RandomState(71)
+    starts, refs = [], []
+    C = None
+    for b in range(B):
+        o = po.OracleSolver(N, dt, is_nonlinear=False, time_integrator="implicit_pc", viscosity=nu,
+                            noise_type=ntype, noise_mag=nm)
+        if ntype == "correlated":
+            C = np.where(o.g.k2 > 0, 1.0 / np.sqrt(np.where(o.g.k2 > 0, o.g.k2, 1.0)), 0.0) + 0j
+            o.set_correlation_function(C)
+        d = len(N)
+        u0 = 0.1 * (rs.standard_normal((d,) + o.g.shape_fourier) + 1j * rs.standard_normal((d,) + o.g.shape_fourier))
+        o.initialize(u0, space="fourier", project=True)
+        starts.append(o.u_hat.copy())
+        nk = int(np.prod(o.g.shape_fourier))
+        noise = [_single_draw_raw(seeds[b], s, nk, o.g.shape_fourier, d, ntype) for s in range(steps)]
+        o.solve(steps, noise=noise)
+        refs.append(o.u_hat)
+    if C is not None:
+        p.set_correlation(_dev(C, p.complex_dtype))
+    uh = _dev(np.stack(starts), p.complex_dtype)
+    p.step(uh, steps)
+    err = po.rel_l2(_host(uh), np.stack(refs))
+    _say(f"resident Stokes {N} {ntype}, device noise restated: rel L2 {err:.3e}")
+    assert err < FP64_TOL
+
+
+# ------------------------------------------- served requests vs the oracle
+def test_served_requests_vs_oracle(mods):
+    """One request of each case, served in FP64 among others, against the
+    oracle: Taylor-Green (RK4, 2/3 rule), forced turbulence (RK3, 3/2 rule,
+    constant-power forcing) and thermal-noise Stokes flow (implicit PC,
+    on-device noise restated for the oracle)."""
+    plan_m, ens_m, workloads = mods
+    from periodicflow import serving
+    S = ens_m.MemberSpec
+    tgv = S(n=(32, 32, 32), dt=2.5e-3, integrator="rk4", viscosity=1 / 1600, dealias="2/3")
+    forced = S(n=(64, 64), dt=5e-3, integrator="rk3", viscosity=0.005, forcing="constant_power", forcing_power=0.1)
+    stokes = S(n=(64, 64), dt=0.05, integrator="implicit_pc", viscosity=1.0, nonlinear=False, noise_mag=0.3)
+    reqs = [
+        serving.Request(0, tgv, 3000, 40, workloads.tgv_host, 0.0),
+        serving.Request(1, forced, 3001, 30, (lambda p: ens_m.random_solenoidal(p, 11, energy=0.5)), 0.0),
+        serving.Request(2, stokes, 3002, 25, workloads.zero_host, 0.0),
+        # neighbours sharing the pools
+        serving.Request(3, forced, 3003, 17, (lambda p: ens_m.random_solenoidal(p, 12, energy=0.5)), 0.001),
+        serving.Request(4, stokes, 3004, 9, workloads.zero_host, 0.001),
+        serving.Request(5, tgv, 3005, 23, workloads.tgv_host, 0.002),
+    ]
+    eng = serving.Engine(serving.GpuBackend("fp64"), hbm_budget=8 << 30, chunk=7, max_slots=4)
+    st = eng.run(reqs)
+    assert st["done"] == len(reqs)
+    shape_of = {r.rid: r for r in reqs}
+    for rid in (0, 1, 2):
+        r = shape_of[rid]
+        sp = r.spec
+        o = po.OracleSolver(sp.n, sp.dt, is_nonlinear=sp.nonlinear, time_integrator=sp.integrator,
+                            viscosity=sp.viscosity, dealias=sp.dealias, noise_mag=sp.noise_mag,
+                            forcing=sp.forcing, forcing_power=sp.forcing_power, forcing_band=sp.forcing_band)
+        like = _PlanShape(sp.n)
+        u0 = r.init(like)
+        o.initialize(u0, space="fourier", project=False, mask_zero_mode=False)
+        if sp.noise_mag:
+            nk = int(np.prod(o.g.shape_fourier))
+            noise = [_single_draw_raw(r.seed, s, nk, o.g.shape_fourier, len(sp.n), sp.noise_type)
+                     for s in range(r.steps)]
+        else:
+            noise = _silent(r.steps)
+        o.solve(r.steps, noise=noise)
+        err = po.rel_l2(r.result, o.u_hat)
+        _say(f"served request {rid} {sp.n} {sp.integrator}: rel L2 vs oracle {err:.3e}")
+        assert err < FP64_TOL

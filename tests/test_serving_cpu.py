@@ -153,3 +153,35 @@ def test_serial_baseline_runs_one_at_a_time():
     assert all(n == 1 for _, n, _ in be.log)
     for r in reqs:
         assert r.result == _alone(r.seed, r.steps)
+
+
+class CostedMock(MockBackend):
+    """The mock with a per-step cost estimate: 1 us per grid point and member."""
+
+    def step_cost(self, spec, n):
+        return 1e-6 * n * int(np.prod(spec.n))
+
+
+def test_srpt_round_takes_the_shortest_pool_first_and_fills_with_others():
+    be = CostedMock()
+    eng = Engine(be, hbm_budget=10**9, chunk=50, max_slots=4, round_s=0.01, overlap=1.25)
+    big = MemberSpec(n=(64, 64), dt=0.01)      # 4096 us per step
+    small = MemberSpec(n=(8, 8), dt=0.01)      # 64 us per step
+    reqs = [Request(0, big, seed=1, steps=3), Request(1, small, seed=2, steps=40),
+            Request(2, small, seed=3, steps=40)]
+    for r in reqs:
+        assert eng._admit(r, 0.0)
+    rnd = eng._round()
+    # remaining work: big 3 x 4096 = 12288 point-steps, small 40 x 64 = 2560, so the
+    # small pool goes first, with round_s / cost = 0.01 / 128e-6 = 78 steps, capped at 40
+    assert rnd[0][0].spec == small and rnd[0][1] == 40
+    # the big pool's chunk (2 steps = 8.2 ms) would exceed 1.25 x 5.1 ms: not in this round
+    assert len(rnd) == 1
+    st = eng.run([], realtime=False)
+    assert st["done"] == 3
+    for r in reqs:
+        assert r.result == _alone(r.seed, r.steps)
+    eng2 = Engine(CostedMock(), hbm_budget=10**9, policy="all", round_s=0.01)
+    for r in [Request(0, big, seed=1, steps=3), Request(1, small, seed=2, steps=40)]:
+        eng2._admit(r, 0.0)
+    assert len(eng2._round()) == 2
