@@ -140,9 +140,12 @@ class GpuBackend:
         return {"plan": p, "state": p.new_state(), "stream": st}
 
     def load(self, h, slot, req):
+        """The request's initial spectrum (a host array, or a device tensor
+        its init callable built on the GPU) into slot ``slot``."""
         p = h["plan"]
         init = req.init(p) if callable(req.init) else req.init
-        h["state"][slot].copy_(self.torch.as_tensor(np.asarray(init), device="cuda", dtype=p.complex_dtype))
+        t = init if isinstance(init, self.torch.Tensor) else self.torch.as_tensor(np.asarray(init))
+        h["state"][slot].copy_(t.to(device="cuda", dtype=p.complex_dtype))
         p.set_member(slot, req.seed, 0)
 
     def move(self, h, src, dst, req):
@@ -291,7 +294,13 @@ class Engine:
         pool.last_used = r.t_done
 
     def _remaining(self, pool):
-        return min((r.steps - r.steps_done) for r in pool.slots) * int(np.prod(pool.spec.n))
+        """Work left before the pool's shortest member finishes: estimated
+        seconds with a backend step_cost, else point-steps."""
+        steps = min((r.steps - r.steps_done) for r in pool.slots)
+        cost = getattr(self.be, "step_cost", None)
+        if cost is not None:
+            return steps * cost(pool.spec, len(pool.slots))
+        return steps * int(np.prod(pool.spec.n))
 
     def _steps_this_round(self, pool):
         k = self.chunk

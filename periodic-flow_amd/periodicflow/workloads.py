@@ -175,18 +175,18 @@ def serving_requests(count=400, rate=20.0, seed=42, max_steps=2000, sizes=None):
         steps = int(round(math.exp(rs.uniform(math.log(100), math.log(max_steps)))))
         if case == "tgv":
             spec = MemberSpec(n=n, dt=2.5e-3, integrator="rk4", viscosity=1 / 1600, dealias="2/3")
-            init = tgv_host
+            init = tgv_device
         elif case == "stokes":
             dx = 2 * np.pi / n[0]
             spec = MemberSpec(n=n, dt=0.05, integrator="implicit_pc", viscosity=1.0, nonlinear=False,
                               noise_mag=math.sqrt(2 * 1e-3 / dx**2))
-            init = zero_host
+            init = zero_device
         else:
             # CFL ~ 0.15 at |u| ~ 2
             dt = min(5e-3, 0.15 * (2 * np.pi / n[0]) / 2.0)
             spec = MemberSpec(n=n, dt=float(f"{dt:.3g}"), integrator="rk3", viscosity=0.005,
                               forcing="constant_power", forcing_power=0.1)
-            init = (lambda p, s=3000 + i: random_solenoidal(p, s, energy=0.5))
+            init = (lambda p, s=3000 + i: solenoidal_device(p, s, energy=0.5))
         out.append(Request(i, spec, 3000 + i, steps, init, t))
     return out
 
@@ -213,3 +213,62 @@ def tgv_host(plan):
 
 def zero_host(plan):
     return np.zeros((plan.dim,) + plan.shape_fourier, dtype=complex)
+
+
+def tgv_device(plan):
+    """tgv_host's spectrum built on the device (no host array to transfer)."""
+    u = torch.zeros((plan.dim,) + plan.shape_fourier, dtype=plan.complex_dtype, device="cuda")
+    n = plan.n
+    if plan.dim == 3:
+        for sy in (1, -1):
+            j = 1 if sy > 0 else n[1] - 1
+            u[0, 1, j, 1] = -0.125j
+            u[0, n[0] - 1, j, 1] = 0.125j
+            u[1, 1, j, 1] = (0.125j if sy > 0 else -0.125j)
+            u[1, n[0] - 1, j, 1] = (0.125j if sy > 0 else -0.125j)
+    else:
+        for sx in (1, -1):
+            i = 1 if sx > 0 else n[0] - 1
+            u[0, i, 1] = -0.25j * sx
+            u[1, i, 1] = 0.25j
+    return u
+
+
+def zero_device(plan):
+    return torch.zeros((plan.dim,) + plan.shape_fourier, dtype=plan.complex_dtype, device="cuda")
+
+
+def solenoidal_device(plan, seed, energy=0.5, kp=4.0):
+    """A seeded solenoidal field with E(k) ~ k^4 exp(-2 k^2 / kp^2) and total
+    energy ``energy``, generated on the device (torch's generator seeded with
+    ``seed``; the same shaping, projection and normalisation as
+    random_solenoidal, whose host draws it does not reproduce)."""
+    d = plan.dim
+    dev = "cuda"
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(int(seed))
+    ks = []
+    for i, ni in enumerate(plan.n):
+        k = torch.fft.fftfreq(ni, 1.0 / ni, device=dev, dtype=torch.float64) if i < d - 1 else \
+            torch.fft.rfftfreq(ni, 1.0 / ni, device=dev, dtype=torch.float64)
+        shp = [1] * d
+        shp[i] = k.numel()
+        ks.append(k.view(shp))
+    shape = plan.shape_fourier
+    k2 = sum(k**2 for k in ks).expand(shape)
+    kk = torch.sqrt(k2)
+    amp = torch.where(kk > 0, kk**2 * torch.exp(-kk**2 / kp**2), torch.zeros_like(kk)) / \
+        torch.clamp(kk, min=1e-30) ** ((d - 1) / 2)
+    re = torch.randn((d,) + shape, generator=gen, device=dev, dtype=torch.float64)
+    im = torch.randn((d,) + shape, generator=gen, device=dev, dtype=torch.float64)
+    u = torch.complex(re, im) * amp
+    p = sum(ks[i] * u[i] for i in range(d)) / torch.where(k2 == 0, torch.ones_like(k2), k2)
+    u = torch.stack([u[i] - ks[i] * p for i in range(d)])
+    w = torch.full(shape, 2.0, device=dev, dtype=torch.float64)
+    w[..., 0] = 1.0
+    if plan.n[-1] % 2 == 0:
+        w[..., -1] = 1.0
+    e = 0.5 * float(torch.sum(w * torch.sum(u.real**2 + u.imag**2, dim=0)))
+    if e > 0:
+        u = u * math.sqrt(energy / e)
+    return u.to(plan.complex_dtype)

@@ -97,6 +97,7 @@ def test_graph_replay_matches_eager(mods):
 
 def test_serving_results_equal_alone(mods):
     ensemble, _, serving = mods
+    from periodicflow import workloads
     _say("serving")
     S = ensemble.MemberSpec
     kinds = [
@@ -108,8 +109,10 @@ def test_serving_results_equal_alone(mods):
     reqs = []
     for i in range(14):
         spec = kinds[i % 3]
-        reqs.append(serving.Request(i, spec, seed=3000 + i, steps=int(rs.randint(5, 40)),
-                                    init=(lambda p, s=i: ensemble.random_solenoidal(p, 900 + s, energy=0.2)),
+        # host-built and device-built initial fields alternate
+        init = ((lambda p, s=i: ensemble.random_solenoidal(p, 900 + s, energy=0.2)) if i % 2 else
+                (lambda p, s=i: workloads.solenoidal_device(p, 900 + s, energy=0.2)))
+        reqs.append(serving.Request(i, spec, seed=3000 + i, steps=int(rs.randint(5, 40)), init=init,
                                     arrival=0.002 * i))
     be = serving.GpuBackend("fp32")
     eng = serving.Engine(be, hbm_budget=8 << 30, chunk=8, max_slots=4)

@@ -172,8 +172,8 @@ def test_srpt_round_takes_the_shortest_pool_first_and_fills_with_others():
     for r in reqs:
         assert eng._admit(r, 0.0)
     rnd = eng._round()
-    # remaining work: big 3 x 4096 = 12288 point-steps, small 40 x 64 = 2560, so the
-    # small pool goes first, with round_s / cost = 0.01 / 128e-6 = 78 steps, capped at 40
+    # estimated time left: big 3 x 4.1 ms, small 40 x 128 us, so the small pool
+    # goes first, with round_s / cost = 0.01 / 128e-6 = 78 steps, capped at 40
     assert rnd[0][0].spec == small and rnd[0][1] == 40
     # the big pool's chunk (2 steps = 8.2 ms) would exceed 1.25 x 5.1 ms: not in this round
     assert len(rnd) == 1
