@@ -1,7 +1,8 @@
 // Register butterflies and the LDS Stockham engine used by every FFT pass.
 //
 // A length-L transform (L = 2^a 3^b) runs as a sequence of radix passes
-// (radix 3 first, then 16/8/4/2).  Pass p with radix R and span Ns (product
+// (make_plan: radix 3 first, then 16/8/4/2, or fewer passes with the mixed
+// radices 6, 9, 12, 24 where the length has a factor 3).  Pass p with radix R and span Ns (product
 // of the earlier radices) reads element j + r L/R, multiplies it by
 // W_{Ns R}^{r (j mod Ns)}, does a radix-R DFT in registers and writes it to
 // (j / Ns) Ns R + (j mod Ns) + r Ns: the Stockham auto-sort order, so the
@@ -19,6 +20,21 @@ struct RadixPlan {
 };
 
 constexpr RadixPlan make_plan(int L) {
+  // Lengths with a factor 3 whose radix-2/3/4/8/16 plan needs three or more
+  // passes: the mixed radices 6, 9, 12 and 24 bring them to two (three for
+  // 1536), one LDS round and barrier pair fewer per transform.
+  switch (L) {
+    case 18: return RadixPlan{2, {9, 2, 0, 0, 0, 0}};
+    case 36: return RadixPlan{2, {9, 4, 0, 0, 0, 0}};
+    case 72: return RadixPlan{2, {9, 8, 0, 0, 0, 0}};
+    case 96: return RadixPlan{2, {6, 16, 0, 0, 0, 0}};
+    case 144: return RadixPlan{2, {9, 16, 0, 0, 0, 0}};
+    case 192: return RadixPlan{2, {12, 16, 0, 0, 0, 0}};
+    case 288: return RadixPlan{2, {12, 24, 0, 0, 0, 0}};
+    case 384: return RadixPlan{2, {24, 16, 0, 0, 0, 0}};
+    case 1536: return RadixPlan{3, {6, 16, 16, 0, 0, 0}};
+    default: break;
+  }
   RadixPlan p{0, {0, 0, 0, 0, 0, 0}};
   int a = 0, b = 0, n = L;
   while (n % 2 == 0) { n /= 2; ++a; }
@@ -137,12 +153,72 @@ template <int S, typename T> __device__ __forceinline__ void dft16(cplx<T>* v) {
   t = v[11]; v[11] = v[14]; v[14] = t;
 }
 
+// cos(2 pi m / 72) for m = 0..18 (a quarter turn); cos72 / sin72 extend it
+// by symmetry to every m.  72 is a multiple of the radices 6, 9, 12, 24.
+constexpr double kCos72[19] = {1.0, 0.996194698091745532295, 0.984807753012208059367, 0.96592582628906828675,
+                               0.939692620785908384054, 0.906307787036649963243, 0.866025403784438646764,
+                               0.819152044288991789684, 0.766044443118978035202, 0.707106781186547524401,
+                               0.642787609686539326323, 0.573576436351046096108, 0.5, 0.422618261740699436187,
+                               0.342020143325668733044, 0.258819045102520762349, 0.173648177666930348852,
+                               0.0871557427476581735581, 0.0};
+__host__ __device__ constexpr double cos72(int m) {
+  m = ((m % 72) + 72) % 72;
+  return m <= 18 ? kCos72[m] : (m <= 36 ? -kCos72[36 - m] : (m <= 54 ? -kCos72[m - 36] : kCos72[72 - m]));
+}
+__host__ __device__ constexpr double sin72(int m) { return cos72(m - 18); }
+
+// multiply by exp(S 2 pi i m / R) for R dividing 72; m is a constant once the
+// caller's loops are unrolled, so the branches and table reads fold away
+template <int S, int R, typename T> __device__ __forceinline__ cplx<T> wr(cplx<T> a, int m) {
+  static_assert(72 % R == 0, "wr: R must divide 72");
+  const int q = ((m % R) + R) % R * (72 / R);
+  if (q == 0) return a;
+  if (q == 18) return mul_si<S, T>(a);
+  if (q == 36) return mk<T>(-a.x, -a.y);
+  if (q == 54) return mul_si<-S, T>(a);
+  const T c = (T)cos72(q), sn = (T)(S * sin72(q));
+  return mk<T>(a.x * c - a.y * sn, a.x * sn + a.y * c);
+}
+
+template <int R, int S, typename T> __device__ __forceinline__ void dft(cplx<T>* v);
+
+// DFT of size R1 R2 by one Cooley-Tukey split: input n = R2 n1 + n2, output
+// k = k1 + R1 k2; inner DFTs of size R1 over n1, twiddles w_R^(n2 k1), outer
+// DFTs of size R2 over n2, results in natural order.
+template <int R1, int R2, int S, typename T> __device__ __forceinline__ void dft_ct(cplx<T>* v) {
+  constexpr int R = R1 * R2;
+  cplx<T> y[R];
+#pragma unroll
+  for (int n2 = 0; n2 < R2; ++n2) {
+    cplx<T> t[R1];
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) t[n1] = v[R2 * n1 + n2];
+    dft<R1, S, T>(t);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) y[n2 * R1 + k1] = wr<S, R, T>(t[k1], n2 * k1);
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) {
+    cplx<T> t[R2];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) t[n2] = y[n2 * R1 + k1];
+    dft<R2, S, T>(t);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) v[k1 + R1 * k2] = t[k2];
+  }
+}
+
 template <int R, int S, typename T> __device__ __forceinline__ void dft(cplx<T>* v) {
   if constexpr (R == 2) dft2<S, T>(v[0], v[1]);
   else if constexpr (R == 3) dft3<S, T>(v);
   else if constexpr (R == 4) dft4<S, T>(v[0], v[1], v[2], v[3]);
   else if constexpr (R == 8) dft8<S, T>(v);
   else if constexpr (R == 16) dft16<S, T>(v);
+  else if constexpr (R == 6) dft_ct<3, 2, S, T>(v);
+  else if constexpr (R == 9) dft_ct<3, 3, S, T>(v);
+  else if constexpr (R == 12) dft_ct<3, 4, S, T>(v);
+  else if constexpr (R == 24) dft_ct<3, 8, S, T>(v);
+  else static_assert(R == 2, "unsupported radix");
 }
 
 // Twiddle table, one block per pass p >= 1 (pass 0 has span 1 and none):
