@@ -155,6 +155,19 @@ int pf_rhs_nonlinear_noalloc(const pf_desc* desc, const void* u_hat, void* rhs, 
  * The step counter advances by n_steps. */
 int pf_step(pf_plan* plan, void* u_hat, int64_t n_steps, const void* noise, void* stream);
 
+/* pf_step of n_steps for several plans at once (an ensemble whose members
+ * have different grids, one plan per grid): every stage issues each pass of
+ * all plans together, and two plans whose transform lengths form a compiled
+ * pair (FP32, default launch variants: 192 + 144, 128 + 96, 144 + 96,
+ * 256 + 192) share one launch of that pass.  Results are those of pf_step on
+ * each plan alone.  Plans that cannot go pass by pass together (Stokes flow,
+ * other integrators or dimension, no fused stage pass, profiling on) step
+ * one after another.  Noise is the on-device stream.  paired (nullable)
+ * receives the number of paired launches issued.  No counterpart in the
+ * reference, which runs one solve per MPI job (simulation.py:205-213). */
+int pf_step_multi(pf_plan* const* plans, void* const* u_hats, int32_t n_plans, int64_t n_steps, int32_t* paired,
+                  void* stream);
+
 /* pf_step of n_steps with the static structure factor accumulated after
  * every step: acc (device doubles, nk) += sum over the active members and
  * components of |u_hat|^2 (pf_accumulate_sk after each step).  Stokes flow

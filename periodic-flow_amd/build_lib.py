@@ -29,6 +29,7 @@ SOURCES = [
     "pf_row_f64.hip",
     "pf_api.hip",
     "pf_point.hip",
+    "pf_pair.hip",
 ]
 
 FLAGS = [

@@ -91,6 +91,7 @@ struct pf_plan {
   bool fuse_stage = true;   // axis-0 forward pass fused with the stage update
   int tsel[PF_NPASS] = {};  // launch variant per pass (pf_set_tiles, pf_autotune)
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[PF_NPASS];
+  struct LaunchLog* rec = nullptr;  // while set, pass launches are recorded instead of issued
 };
 
 namespace {
@@ -116,6 +117,25 @@ struct PassTimer {
     }
   }
 };
+
+}  // namespace
+
+// The launches of one stage's nonlinear passes of a plan, recorded (not
+// issued) so that several plans' passes can be issued together.
+struct LaunchRec {
+  bool row;
+  int L, kind, v;
+  ColArgs ca;
+  RowArgs ra;
+  const void* tw;
+};
+struct LaunchLog {
+  LaunchRec r[8];
+  int n = 0;
+  bool overflow = false;
+};
+
+namespace {
 
 int dev_alloc(pf_plan* p, void** ptr, size_t bytes) {
   if (bytes == 0) bytes = 256;
@@ -205,7 +225,26 @@ StageArgs geometry(const pf_plan* p) {
 
 // v: the launch variant (0 = default; the autotuner's choice per pass)
 template <typename T> int row(const pf_plan* p, int L, int mode, int v, const RowArgs& a, hipStream_t s) {
+  if (p->rec) {
+    LaunchLog& g = *p->rec;
+    if (g.n >= 8) { g.overflow = true; return 0; }
+    LaunchRec& r = g.r[g.n++];
+    r.row = true; r.L = L; r.kind = mode; r.v = v; r.ra = a; r.tw = tw_of(p, L);
+    return 0;
+  }
   return launch_row<T>(L, mode, a, tw_of(p, L), s, v);
+}
+
+// A column pass launch, or its record while the plan records.
+template <typename T> int emit_col(pf_plan* p, int L, int kind, const ColArgs& a, hipStream_t s, int v) {
+  if (p->rec) {
+    LaunchLog& g = *p->rec;
+    if (g.n >= 8) { g.overflow = true; return 0; }
+    LaunchRec& r = g.r[g.n++];
+    r.row = false; r.L = L; r.kind = kind; r.v = v; r.ca = a; r.tw = tw_of(p, L);
+    return 0;
+  }
+  return launch_col<T>(L, kind, a, tw_of(p, L), s, v);
 }
 
 ColArgs col_args() {
@@ -259,7 +298,7 @@ int col_groups(pf_plan* p, int L, int kind, int v, ColArgs a, int ngroups, long 
   g.out_lp = out_lp;
   g.n_outer = 1;
   g.in_outer = g.out_outer = 0;
-  return launch_col<T>(L, kind, a, tw_of(p, L), s, v);
+  return emit_col<T>(p, L, kind, a, s, v);
 }
 
 // The same over the last-axis columns l of n_outer per-plane matrices (the
@@ -278,7 +317,7 @@ int col_planes(pf_plan* p, int L, int kind, int v, ColArgs a, int n_outer, long 
   g.n_outer = n_outer;
   g.in_outer = in_outer;
   g.out_outer = out_outer;
-  return launch_col<T>(L, kind, a, tw_of(p, L), s, v);
+  return emit_col<T>(p, L, kind, a, s, v);
 }
 
 // Inverse passes, the fused last-axis pass and (3D) the axis-1 forward pass
@@ -687,6 +726,110 @@ template <typename T> int step_impl(pf_plan* p, void* uhat, long long n_steps, c
     }
     PF_TRY(launch_member_step(p->msteps, p->nact, s));
     p->step += 1;
+  }
+  return 0;
+}
+
+constexpr int kMaxMulti = 16;  // plans one pf_step_multi call advances together
+
+// Issues the recorded launches of several plans pass by pass (record r of
+// every plan before record r + 1 of any, all on one stream, so each plan's
+// passes keep their order): two plans whose records at r are the same pass
+// kind at different compiled lengths, both at the default launch variant,
+// share one paired launch; the rest launch one by one.
+template <typename T> int issue_logs(LaunchLog* logs, int np, hipStream_t s, int* paired) {
+  const int nr = logs[0].n;
+  for (int r = 0; r < nr; ++r) {
+    bool used[kMaxMulti] = {};
+    for (int i = 0; i < np; ++i) {
+      if (used[i]) continue;
+      const LaunchRec& a = logs[i].r[r];
+      int partner = -1;
+      if (sizeof(T) == 4 && a.v == 0) {
+        for (int j = i + 1; j < np && partner < 0; ++j) {
+          const LaunchRec& b = logs[j].r[r];
+          if (!used[j] && b.v == 0 && b.row == a.row && b.kind == a.kind && b.L != a.L && pair_compiled(a.L, b.L))
+            partner = j;
+        }
+      }
+      int rc = -4;
+      if (partner >= 0) {
+        const LaunchRec& b = logs[partner].r[r];
+        rc = a.row ? launch_row_pair<T>(a.kind, a.L, a.ra, a.tw, b.L, b.ra, b.tw, s)
+                   : launch_col_pair<T>(a.kind, a.L, a.ca, a.tw, b.L, b.ca, b.tw, s);
+        if (rc == 0) {
+          used[partner] = true;
+          if (paired) ++*paired;
+        }
+      }
+      if (rc == -4) rc = a.row ? launch_row<T>(a.L, a.kind, a.ra, a.tw, s, a.v) : launch_col<T>(a.L, a.kind, a.ca, a.tw, s, a.v);
+      if (rc) return launch_fail(rc, "issue_logs");
+      used[i] = true;
+    }
+  }
+  return 0;
+}
+
+// n_steps steps of several plans, their nonlinear passes issued together
+// (issue_logs).  Plans that cannot go pass by pass together (other stage
+// counts or dimension, Stokes flow, no fused stage pass) step one after
+// another instead.  Noise is the on-device stream.
+template <typename T> int step_multi_impl(pf_plan** ps, void** us, int np, long long n_steps, hipStream_t s,
+                                          int* paired) {
+  bool together = np >= 2;
+  for (int i = 0; i < np && together; ++i) {
+    const pf_plan* p = ps[i];
+    const bool fused = p->fuse_stage &&
+                       tile_cols<T>(p->M[0], p->dim == 3 ? K_FWD_STAGE3 : K_FWD_STAGE2, p->tsel[PF_PASS_FWD0_STAGE]) > 0;
+    together = p->d.nonlinear && fused && p->d.integrator == ps[0]->d.integrator && p->dim == ps[0]->dim && !p->prof;
+  }
+  if (!together) {
+    for (int i = 0; i < np; ++i) {
+      int rc = step_impl<T>(ps[i], us[i], n_steps, nullptr, s);
+      if (rc) return rc;
+    }
+    return 0;
+  }
+  const int ns = n_stages(ps[0]->d.integrator);
+  for (long long st = 0; st < n_steps; ++st) {
+    for (int stage = 0; stage < ns; ++stage) {
+      LaunchLog logs[kMaxMulti];
+      for (int i = 0; i < np; ++i) {
+        pf_plan* p = ps[i];
+        if (p->d.forcing == PF_FORCING_CONSTANT_POWER) {
+          int rc = band_energy<T>(p, us[i], s);
+          if (rc) return rc;
+        }
+        const bool noise_on = p->d.noise_mag != 0.0 && uses_noise(p->d.integrator);
+        StageArgs a = stage_args(p, us[i], stage, nullptr, noise_on);
+        a.rhs_nl = nullptr;
+        p->rec = &logs[i];
+        int rc = nonlinear_front<T>(p, us[i], s);
+        if (!rc) rc = last_forward<T>(p, p->dim == 3 ? K_FWD_STAGE3 : K_FWD_STAGE2, nullptr, &a, s);
+        p->rec = nullptr;
+        if (rc) return rc;
+        if (logs[i].overflow) return fail(PF_EINVAL, "step_multi: more launches per stage than recorded");
+      }
+      bool same = true;
+      for (int i = 1; i < np; ++i) {
+        same = same && logs[i].n == logs[0].n;
+        for (int r = 0; same && r < logs[0].n; ++r)
+          same = logs[i].r[r].row == logs[0].r[r].row && logs[i].r[r].kind == logs[0].r[r].kind;
+      }
+      if (same) {
+        int rc = issue_logs<T>(logs, np, s, paired);
+        if (rc) return rc;
+      } else {
+        for (int i = 0; i < np; ++i) {
+          int rc = issue_logs<T>(&logs[i], 1, s, nullptr);
+          if (rc) return rc;
+        }
+      }
+    }
+    for (int i = 0; i < np; ++i) {
+      PF_TRY(launch_member_step(ps[i]->msteps, ps[i]->nact, s));
+      ps[i]->step += 1;
+    }
   }
   return 0;
 }
@@ -1116,6 +1259,30 @@ int pf_step(pf_plan* p, void* uhat, int64_t n_steps, const void* noise, void* st
   if (p->nact == 0 || n_steps == 0) return PF_OK;
   return p->fp64 ? step_impl<double>(p, uhat, n_steps, noise, (hipStream_t)stream)
                  : step_impl<float>(p, uhat, n_steps, noise, (hipStream_t)stream);
+}
+
+int pf_step_multi(pf_plan* const* plans, void* const* u_hats, int32_t n_plans, int64_t n_steps, int32_t* paired,
+                  void* stream) {
+  if (paired) *paired = 0;
+  if (!plans || !u_hats || n_plans < 1 || n_plans > kMaxMulti || n_steps < 0)
+    return failf(PF_EINVAL, "pf_step_multi: invalid argument (1 <= n_plans <= %d)", kMaxMulti);
+  pf_plan* ps[kMaxMulti];
+  void* us[kMaxMulti];
+  int np = 0;
+  for (int i = 0; i < n_plans; ++i) {
+    if (!plans[i] || !u_hats[i]) return fail(PF_EINVAL, "pf_step_multi: null plan or state");
+    if (plans[i]->fp64 != plans[0]->fp64) return fail(PF_EINVAL, "pf_step_multi: plans of different precision");
+    for (int j = 0; j < i; ++j)
+      if (plans[j] == plans[i]) return fail(PF_EINVAL, "pf_step_multi: a plan appears twice");
+    if (plans[i]->nact == 0) continue;
+    ps[np] = plans[i];
+    us[np] = u_hats[i];
+    ++np;
+  }
+  if (np == 0 || n_steps == 0) return PF_OK;
+  hipStream_t s = (hipStream_t)stream;
+  return plans[0]->fp64 ? step_multi_impl<double>(ps, us, np, n_steps, s, paired)
+                        : step_multi_impl<float>(ps, us, np, n_steps, s, paired);
 }
 
 int pf_step_accumulate_sk(pf_plan* p, void* uhat, int64_t n_steps, double* acc, void* stream) {

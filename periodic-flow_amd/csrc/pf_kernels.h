@@ -83,22 +83,29 @@ template <typename T, int MODE> constexpr int row_pairs_v(int L, int V) {
   return lds <= 160 * 1024 ? r : 0;
 }
 
+// LDS of a row pass block: the twiddle table, then NP pairs of NS slots
+template <typename T, int L, int MODE, int NP> constexpr int row_lds_tw() { return tw_alloc(L); }
+template <typename T, int L, int MODE, int NP> constexpr int row_lds_buf() {
+  return NP * row_slots<MODE>() * row_pitch(L);
+}
+
+// One block of a row pass: block (bx, by) of its launch; tws and buf are the
+// block's LDS (row_lds_tw and row_lds_buf entries).
 template <typename T, int L, int MODE, int NP>
-__global__ void __launch_bounds__(kThreads) row_pass(RowArgs a, const cplx<T>* __restrict__ tw) {
+__device__ __forceinline__ void row_body(const RowArgs& a, const cplx<T>* __restrict__ tw, cplx<T>* tws,
+                                         cplx<T>* buf, int bx, int by) {
   constexpr int NS = row_slots<MODE>();
   constexpr int LP = row_pitch(L);
-  __shared__ cplx<T> tws[tw_alloc(L)];
-  __shared__ cplx<T> buf[NP * NS * LP];
   // the table size as a constant (evaluated at run time, the plan's radix
   // array would live in private memory)
   constexpr int TWN = tw_size(L);
   for (int m = threadIdx.x; m < TWN; m += kThreads) tws[m] = tw[m];
 
   const int npairs = a.n_rows / 2;
-  const int pair0 = blockIdx.x * NP;
+  const int pair0 = bx * NP;
   int f = 0, b;
-  if (MODE == ROW_C2R || MODE == ROW_R2C) { f = blockIdx.y % a.n_fields; b = blockIdx.y / a.n_fields; }
-  else b = blockIdx.y;
+  if (MODE == ROW_C2R || MODE == ROW_R2C) { f = by % a.n_fields; b = by / a.n_fields; }
+  else b = by;
   auto ix = [](int slot, int e) { return slot * LP + e + (e >> 4); };
   const int N = a.N;
 
@@ -304,6 +311,13 @@ __global__ void __launch_bounds__(kThreads) row_pass(RowArgs a, const cplx<T>* _
     oa[k] = scl<T>(A, s);
     ob[k] = scl<T>(B, s);
   }
+}
+
+template <typename T, int L, int MODE, int NP>
+__global__ void __launch_bounds__(kThreads) row_pass(RowArgs a, const cplx<T>* __restrict__ tw) {
+  __shared__ cplx<T> tws[row_lds_tw<T, L, MODE, NP>()];
+  __shared__ cplx<T> buf[row_lds_buf<T, L, MODE, NP>()];
+  row_body<T, L, MODE, NP>(a, tw, tws, buf, blockIdx.x, blockIdx.y);
 }
 
 // -------------------------------------------------------------- random numbers
@@ -1080,8 +1094,18 @@ __device__ __forceinline__ long long xcd_order(long long bid, long long G) {
 // a bool as a type, to select a lambda's code path at compile time
 template <bool B> struct BoolC { static constexpr bool value = B; };
 
+template <int KIND> constexpr int col_fields() {
+  return KIND == K_FWD_STAGE3 ? 3 : (KIND == K_FWD_STAGE2 ? 2 : 1);
+}
+// LDS of a column pass block: the twiddle table, then L rows of the tile
+template <typename T, int L, int TC, int KIND> constexpr int col_lds_tw() { return tw_alloc(L); }
+template <typename T, int L, int TC, int KIND> constexpr int col_lds_buf() { return L * col_fields<KIND>() * TC; }
+
+// One block of a column pass: block bid of the nblk blocks of its launch;
+// tws and buf are the block's LDS (col_lds_tw and col_lds_buf entries).
 template <typename T, int L, int TC, int KIND, int PIPE, int NT>
-__global__ void __launch_bounds__(NT) col_pass(ColArgs a, const cplx<T>* __restrict__ tw) {
+__device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __restrict__ tw, cplx<T>* tws,
+                                         cplx<T>* buf, long long bid, long long nblk) {
   constexpr bool STAGE = KIND == K_FWD_STAGE3 || KIND == K_FWD_STAGE2;
   constexpr bool INV = KIND == K_INV || KIND == K_INV_UC3 || KIND == K_INV_UC2;
   constexpr bool UC = KIND == K_INV_UC3 || KIND == K_INV_UC2;
@@ -1092,13 +1116,11 @@ __global__ void __launch_bounds__(NT) col_pass(ColArgs a, const cplx<T>* __restr
   constexpr int NIT = (L * W + NT - 1) / NT;
   constexpr int CH = NIT < 16 ? NIT : 16;  // loads kept in flight per thread
   static_assert(NT % TC == 0, "tile width must divide the block size");
-  __shared__ cplx<T> tws[tw_alloc(L)];
-  __shared__ cplx<T> buf[L * W];
   constexpr int TWN = tw_size(L);  // a constant, as in row_pass
   for (int m = threadIdx.x; m < TWN; m += NT) tws[m] = tw[m];
 
   const int nfg = STAGE ? 1 : a.n_fields;
-  long long lg = xcd_order(blockIdx.x, gridDim.x);
+  long long lg = xcd_order(bid, nblk);
   const ColGeom& gm = a.g;
   // PIPE consecutive tiles of one (field, matrix, member) per block; the
   // tile index varies fastest, so the blocks an XCD runs together cover
@@ -1492,6 +1514,64 @@ __global__ void __launch_bounds__(NT) col_pass(ColArgs a, const cplx<T>* __restr
       __syncthreads();
     }
   }
+  }
+}
+
+
+template <typename T, int L, int TC, int KIND, int PIPE, int NT>
+__global__ void __launch_bounds__(NT) col_pass(ColArgs a, const cplx<T>* __restrict__ tw) {
+  __shared__ cplx<T> tws[col_lds_tw<T, L, TC, KIND>()];
+  __shared__ cplx<T> buf[col_lds_buf<T, L, TC, KIND>()];
+  col_body<T, L, TC, KIND, PIPE, NT>(a, tw, tws, buf, blockIdx.x, gridDim.x);
+}
+
+
+// ---------------------------------------------------------- paired launches
+// Two plans' launches of one pass kind in one grid, for ensembles whose
+// members have different transform lengths: blocks [0, split) run segment 0
+// at length L0, blocks [split, ...) segment 1 at L1, each exactly as its own
+// launch would (the same body, arguments and block numbering).  split is
+// segment 0's block count rounded up to a multiple of 8, so that both
+// segments keep their XCD-aware tile order.  LDS is sized for the larger.
+struct ColSeg {
+  ColArgs a;
+  const void* tw;
+  long long nblk;
+};
+
+template <typename T, int L0, int TC0, int L1, int TC1, int KIND>
+__global__ void __launch_bounds__(kThreads) col_pass_pair(ColSeg s0, ColSeg s1, long long split) {
+  constexpr int TW0 = col_lds_tw<T, L0, TC0, KIND>(), TW1 = col_lds_tw<T, L1, TC1, KIND>();
+  constexpr int BF0 = col_lds_buf<T, L0, TC0, KIND>(), BF1 = col_lds_buf<T, L1, TC1, KIND>();
+  __shared__ cplx<T> tws[TW0 > TW1 ? TW0 : TW1];
+  __shared__ cplx<T> buf[BF0 > BF1 ? BF0 : BF1];
+  const long long bid = blockIdx.x;
+  if (bid < split) {
+    if (bid < s0.nblk) col_body<T, L0, TC0, KIND, 1, kThreads>(s0.a, (const cplx<T>*)s0.tw, tws, buf, bid, s0.nblk);
+  } else {
+    col_body<T, L1, TC1, KIND, 1, kThreads>(s1.a, (const cplx<T>*)s1.tw, tws, buf, bid - split, s1.nblk);
+  }
+}
+
+struct RowSeg {
+  RowArgs a;
+  const void* tw;
+  int gx, gy;
+};
+
+template <typename T, int L0, int NP0, int L1, int NP1, int MODE>
+__global__ void __launch_bounds__(kThreads) row_pass_pair(RowSeg s0, RowSeg s1, long long split) {
+  constexpr int TW0 = row_lds_tw<T, L0, MODE, NP0>(), TW1 = row_lds_tw<T, L1, MODE, NP1>();
+  constexpr int BF0 = row_lds_buf<T, L0, MODE, NP0>(), BF1 = row_lds_buf<T, L1, MODE, NP1>();
+  __shared__ cplx<T> tws[TW0 > TW1 ? TW0 : TW1];
+  __shared__ cplx<T> buf[BF0 > BF1 ? BF0 : BF1];
+  const long long bid = blockIdx.x;
+  if (bid < split) {
+    if (bid < (long long)s0.gx * s0.gy)
+      row_body<T, L0, MODE, NP0>(s0.a, (const cplx<T>*)s0.tw, tws, buf, (int)(bid % s0.gx), (int)(bid / s0.gx));
+  } else {
+    const long long b1 = bid - split;
+    row_body<T, L1, MODE, NP1>(s1.a, (const cplx<T>*)s1.tw, tws, buf, (int)(b1 % s1.gx), (int)(b1 / s1.gx));
   }
 }
 

@@ -78,6 +78,8 @@ _SIGS = {
     "pf_rhs_nonlinear_noalloc": (C.c_int, [C.POINTER(PfDesc), _P, _P, _P]),
     "pf_step": (C.c_int, [_P, _P, C.c_int64, _P, _P]),
     "pf_step_accumulate_sk": (C.c_int, [_P, _P, C.c_int64, _P, _P]),
+    "pf_step_multi": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int32, C.c_int64,
+                                C.POINTER(C.c_int32), _P]),
     "pf_compute_rhs": (C.c_int, [_P, _P, C.c_double, _P, _P, _P, _P, _P, _P]),
     "pf_stage_update": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P, _P, _P, _P]),
     "pf_draw_noise": (C.c_int, [_P, _P, _P]),

@@ -1,11 +1,14 @@
 """Batched ensembles: many independent members, of uneven grid sizes, on one GPU.
 
 Members with the same shape and parameters share one plan (one launch per
-pass for all of them); distinct shapes form separate groups that are
-launched back to back on one stream, or replayed from one captured HIP graph
-per step.  Every member keeps its own seed and RNG step counter, so its
-result does not depend on the other members.  The reference runs one solve
-per MPI job; this layer has no counterpart there.
+pass for all of them).  Groups of different shapes advance together through
+pf_step_multi: every stage issues each pass of all groups at once, and two
+groups whose transform lengths form a compiled pair (e.g. 128^3 and 96^3
+under the 3/2 rule: 192 and 144) share one launch of that pass.  A step can
+also be captured into one HIP graph and replayed.  Every member keeps its
+own seed and RNG step counter, so its result does not depend on the other
+members.  The reference runs one solve per MPI job; this layer has no
+counterpart there.
 """
 from __future__ import annotations
 
@@ -15,7 +18,7 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
-from .plan import Plan
+from .plan import Plan, step_multi
 
 
 @dataclass(frozen=True)
@@ -108,10 +111,14 @@ class Ensemble:
         return {k: g.plan.autotune(g.state, reps) for k, g in self.groups.items()}
 
     def step(self, n_steps=1):
+        """Advance every member n_steps; returns the number of paired launches."""
         if self._tune:
             self.autotune()
-        for g in self.groups.values():
-            g.plan.step(g.state, n_steps)
+        gs = list(self.groups.values())
+        if len(gs) == 1:
+            gs[0].plan.step(gs[0].state, n_steps)
+            return 0
+        return step_multi([g.plan for g in gs], [g.state for g in gs], n_steps)
 
     # ---------------------------------------------------------------- graphs
     def capture(self):

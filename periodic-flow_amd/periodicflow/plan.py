@@ -320,5 +320,22 @@ class Plan:
         call("pf_accumulate_sk", self._h, _ptr(u_hat), _ptr(acc), _stream())
 
 
+def step_multi(plans, states, n_steps=1):
+    """Advance several plans (an ensemble's groups of different grids) by
+    n_steps with their passes issued together, pairs of compiled transform
+    lengths sharing one launch (pf_step_multi); returns the number of paired
+    launches.  Results are those of each plan's own step()."""
+    if len(plans) != len(states):
+        raise ValueError("one state per plan")
+    for p, st in zip(plans, states):
+        p._check(st, p.state_shape(), p.complex_dtype, "u_hat")
+    n = len(plans)
+    hs = (C.c_void_p * n)(*[p._h.value for p in plans])
+    us = (C.c_void_p * n)(*[st.data_ptr() for st in states])
+    paired = C.c_int32(0)
+    call("pf_step_multi", hs, us, n, int(n_steps), C.byref(paired), _stream())
+    return int(paired.value)
+
+
 def version():
     return _lib.load().pf_version().decode()

@@ -194,3 +194,36 @@ def test_register_resident_stokes_equals_per_step(mods, N, ntype, prec, B):
     rel = float(torch.max(torch.abs(acc_r - acc_s) / acc_s.clamp_min(1e-300)))
     _say("   S(k) accumulation max rel diff", rel)
     assert rel < 1e-12
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_paired_launches_equal_separate_plans(mods, prec):
+    """An ensemble of a 128^3 and a 96^3 group (3/2 rule, RK3, thermal noise,
+    forcing): every nonlinear pass of both groups in one paired launch (FP32;
+    FP64 has no paired kernels and steps the groups in turn), results bitwise
+    those of each plan stepped alone."""
+    ensemble, plan, _ = mods
+    _say("paired launches", prec)
+    S = ensemble.MemberSpec
+    members = [(S(n=(128,) * 3, dt=5e-3, integrator="rk3", viscosity=0.005, noise_mag=0.01,
+                  forcing="constant_power"), 21),
+               (S(n=(96,) * 3, dt=5e-3, integrator="rk3", viscosity=0.005, noise_mag=0.01,
+                  forcing="constant_power"), 22),
+               (S(n=(96,) * 3, dt=5e-3, integrator="rk3", viscosity=0.005, noise_mag=0.01,
+                  forcing="constant_power"), 23)]
+    ens = ensemble.Ensemble(members, precision=prec)
+    inits = []
+    for i, (spec, seed) in enumerate(members):
+        u = ensemble.random_solenoidal(ens.plan_of(i), 40 + i, energy=0.5)
+        inits.append(u)
+        ens.set_fourier(i, u)
+    paired = ens.step(3)
+    # 3 steps x 3 stages x 5 passes, all paired in FP32
+    assert paired == (45 if prec == "fp32" else 0), paired
+    for i, (spec, seed) in enumerate(members):
+        p = plan.Plan(spec.n, spec.dt, precision=prec, integrator="rk3", viscosity=spec.viscosity,
+                      noise_mag=spec.noise_mag, forcing=spec.forcing, seeds=[seed])
+        st = torch.as_tensor(inits[i][None], device="cuda", dtype=p.complex_dtype).contiguous()
+        p.step(st, 3)
+        assert torch.equal(st[0], ens.state(i)), i
+        assert ens.plan_of(i).step_counter == 3
