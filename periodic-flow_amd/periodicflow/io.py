@@ -315,92 +315,63 @@ class HDF5Writer:
             self._h5.close(self._fid)
             self._fid = None
 
-    @staticmethod
-    def _distribute_steps(rank: int, size: int, steps: list):
-        """io.py:49-54: round-robin share of the steps."""
-        return [steps[i] for i in range(len(steps)) if i % size == rank]
-
-    def _get_dataset_info(self, input_file: Path):
-        """io.py:56-95: variables, dimension key, steps and the output shape."""
+    def _scan(self, fid):
+        """What the run's file holds: its variables (sorted), the group of
+        step datasets ('2D' or '3D'), its dimension and the steps in
+        numeric order."""
         h5 = self._h5
-        fid = h5.open(input_file)
-        try:
-            variables = h5.members(fid)
-            inner = h5.members(fid, variables[0])
-            str_dim = inner[0] if inner else None
-            ndims = {"3d": 3, "2d": 2}.get(str(str_dim).lower(), 0)
-            if ndims == 0:
-                raise ValueError(f"get_dataset_info: Dimension identifier '{str_dim}' not recognized.")
-            steps = sorted(h5.members(fid, f"{variables[0]}/{str_dim}"), key=int)
-            shape = [h5.read(fid, f"{variables[0]}/mesh/{key}").size + (1 if self.periodic else 0)
-                     for key in ["x0", "x1", "x2"][:ndims]]
-        finally:
-            h5.close(fid)
-        return {
-            "variables": variables,
-            "str_dim": str_dim,
-            "ndims": ndims,
-            "shape": shape,
-            "steps": steps,
-            "n_digits": len(str(steps[-1])),
-            "local_steps": HDF5Writer._distribute_steps(self.mpi_rank, self.mpi_size, steps),
-        }
-
-    def _prepare_temp_file(self, input_file: Path, temp_file: Path, dataset_info: dict):
-        """io.py:97-140: coordinates, times and empty step datasets."""
-        h5 = self._h5
-        fr = h5.open(input_file)
-        try:
-            fw = h5.create(temp_file)
-            try:
-                var0 = dataset_info["variables"][0]
-                ndims = dataset_info["ndims"]
-                for key, label, x_end in zip(["x0", "x1", "x2"][:ndims], ["x", "y", "z"][:ndims],
-                                             self.end_points[:ndims]):
-                    coord = h5.read(fr, f"{var0}/mesh/{key}")
-                    if self.periodic:
-                        coord = np.append(coord, x_end)
-                    h5.create_dataset(fw, label, coord.shape, coord)
-                steps = dataset_info["steps"]
-                t = np.linspace(self.time_points[0], self.time_points[-1], len(steps))
-                h5.create_dataset(fw, "t", t.shape, t)
-                n_digits = dataset_info["n_digits"]
-                for var in dataset_info["variables"]:
-                    h5.require_group(fw, var)
-                    for step in steps:
-                        h5.create_dataset(fw, f"{var}/{str(step).zfill(n_digits)}", dataset_info["shape"])
-            finally:
-                h5.close(fw)
-        finally:
-            h5.close(fr)
+        names = h5.members(fid)
+        if not names:
+            raise ValueError(f"{self.path}: no variables to reconfigure")
+        kids = [k for k in h5.members(fid, names[0]) if k != "mesh"]
+        key = kids[0] if kids else ""
+        dim = {"2d": 2, "3d": 3}.get(key.lower())
+        if dim is None:
+            raise ValueError(f"{self.path}: '{names[0]}' holds no 2D or 3D step group (found {kids})")
+        steps = sorted((int(k) for k in h5.members(fid, f"{names[0]}/{key}")))
+        return names, key, dim, steps
 
     def reconfigure_dataset(self):
-        """io.py:142-196: rewrite the run's file into the flat layout (through
-        a ``_temp.h5`` file that then replaces it)."""
+        """io.py:142-196: rewrite the run's file into the flat layout.
+
+        One pass from the run's file into ``<stem>_temp.h5`` beside it, which
+        then replaces it: per axis i the coordinates as root datasets 'x',
+        'y'[, 'z'] (with the end point ``end_points[i]`` appended when
+        ``periodic``), 't' = linspace(first, last time point, number of
+        snapshots), and per variable one float64 dataset per snapshot named
+        by its step padded with zeros to the width of the last step (with the
+        first plane repeated at the end of every axis when ``periodic``)."""
         self.close()
-        logger.info("HDF5Writer.reconfigure_dataset: Start reconfiguring dataset.")
-        input_file = self.path
-        # <stem>_temp.h5 beside it (io.py:155; only the file's own name changes)
-        temp_file = input_file.with_name(input_file.stem + "_temp.h5")
-        if not input_file.exists():
-            raise FileNotFoundError(f"Solution file '{input_file}' does not exist.")
-        info = self._get_dataset_info(input_file)
-        self._prepare_temp_file(input_file, temp_file, info)
+        src_path = self.path
+        if not src_path.exists():
+            raise FileNotFoundError(f"Solution file '{src_path}' does not exist.")
+        tmp_path = src_path.with_name(src_path.stem + "_temp.h5")
+        logger.info(f"reconfiguring {src_path} into the flat snapshot layout")
         h5 = self._h5
-        fr = h5.open(input_file)
+        src = h5.open(src_path)
         try:
-            fw = h5.open(temp_file, write=True)
+            names, key, dim, steps = self._scan(src)
+            width = len(str(steps[-1])) if steps else 1
+            dst = h5.create(tmp_path)
             try:
-                for var in info["variables"]:
-                    for step in info["local_steps"]:
-                        data = h5.read(fr, f"{var}/{info['str_dim']}/{step}")
+                for i, label in enumerate("xyz"[:dim]):
+                    c = h5.read(src, f"{names[0]}/mesh/x{i}")
+                    if self.periodic:
+                        c = np.append(c, self.end_points[i])
+                    h5.create_dataset(dst, label, c.shape, c)
+                t = np.linspace(self.time_points[0], self.time_points[-1], len(steps))
+                h5.create_dataset(dst, "t", t.shape, t)
+                for name in names:
+                    h5.require_group(dst, name)
+                    for st in steps:
+                        a = h5.read(src, f"{name}/{key}/{st}")
                         if self.periodic:
-                            data = periodic_bc(data)
-                        h5.write_dataset(fw, f"{var}/{str(step).zfill(info['n_digits'])}", data)
+                            a = periodic_bc(a)
+                        h5.create_dataset(dst, f"{name}/{st:0{width}d}", a.shape, a)
             finally:
-                h5.close(fw)
+                h5.close(dst)
         finally:
-            h5.close(fr)
-        input_file.unlink()
-        temp_file.rename(input_file)
-        logger.info("HDF5Writer.reconfigure_dataset: Dataset reconfigured successfully.")
+            h5.close(src)
+        src_path.unlink()
+        tmp_path.rename(src_path)
+        logger.info(f"{src_path}: {len(names)} variable(s), {len(steps)} snapshot(s) written in the flat layout")

@@ -182,9 +182,3 @@ def test_reconfigure_missing_file(h5, tmp_path):
     w.path.unlink()
     with pytest.raises(FileNotFoundError):
         w.reconfigure_dataset()
-
-
-def test_distribute_steps():
-    # io.py:49-54
-    assert pio.HDF5Writer._distribute_steps(0, 1, ["0", "5"]) == ["0", "5"]
-    assert pio.HDF5Writer._distribute_steps(1, 2, ["0", "5", "10"]) == ["5"]
