@@ -79,8 +79,9 @@ struct pf_plan {
   long long* msteps = nullptr;   // per-member RNG step counters (device)
   int nact = 1;                  // members [0, nact) are advanced
   double* fcoef = nullptr;
-  // twiddle tables of the plan's distinct transform lengths (at most 2 per axis)
-  struct Tw { int L; void* ptr; } tw[6] = {};
+  // twiddle tables of the plan's distinct transform lengths (at most 2 per
+  // axis) and radix plans (make_plan: 0, and 1 where the length has one)
+  struct Tw { int L, rp; void* ptr; } tw[12] = {};
   int ntw = 0;
   long long step = 0;
   size_t bytes = 0;
@@ -163,22 +164,26 @@ bool supported_length(int L) {
   }
 }
 
-// Twiddle table of length L (the plan holds one per distinct length).
-const void* tw_of(const pf_plan* p, int L) {
+// Twiddle table of length L under radix plan rp.
+const void* tw_of(const pf_plan* p, int L, int rp = 0) {
   for (int i = 0; i < p->ntw; ++i)
-    if (p->tw[i].L == L) return p->tw[i].ptr;
+    if (p->tw[i].L == L && p->tw[i].rp == rp) return p->tw[i].ptr;
   return nullptr;
 }
 
-int make_twiddles(pf_plan* p, int L) {
-  if (tw_of(p, L)) return 0;
-  if (p->ntw >= 6) return fail(PF_EINVAL, "too many transform lengths");
+int make_twiddles(pf_plan* p, int L, int rp = 0) {
+  if (rp == 0 && has_alt_plan(L)) {
+    int rc = make_twiddles(p, L, 1);
+    if (rc) return rc;
+  }
+  if (tw_of(p, L, rp)) return 0;
+  if (p->ntw >= 12) return fail(PF_EINVAL, "too many transform lengths");
   void* dptr = nullptr;
   int rc = dev_alloc(p, &dptr, (size_t)L * p->csz);
   if (rc) return rc;
   // per-pass blocks: entry tw_offset(L, p) + (r - 1) Ns + k = exp(-2 pi i r k / (Ns R))
   const long double twopi = 6.283185307179586476925286766559005768L;
-  const RadixPlan pl = make_plan(L);
+  const RadixPlan pl = make_plan(L, rp);
   std::vector<long double> re, im;
   int ns = 1;
   for (int q = 0; q < pl.n; ++q) {
@@ -193,7 +198,7 @@ int make_twiddles(pf_plan* p, int L) {
     }
     ns *= R;
   }
-  if ((int)re.size() != tw_size(L)) return fail(PF_EINVAL, "twiddle table size mismatch");
+  if ((int)re.size() != tw_size(L, rp)) return fail(PF_EINVAL, "twiddle table size mismatch");
   if (p->fp64) {
     std::vector<double> h(2 * re.size() + 2, 0.0);
     for (size_t m = 0; m < re.size(); ++m) { h[2 * m] = (double)re[m]; h[2 * m + 1] = (double)im[m]; }
@@ -203,7 +208,7 @@ int make_twiddles(pf_plan* p, int L) {
     for (size_t m = 0; m < re.size(); ++m) { h[2 * m] = (float)re[m]; h[2 * m + 1] = (float)im[m]; }
     PF_HIP(hipMemcpy(dptr, h.data(), 2 * re.size() * sizeof(float), hipMemcpyHostToDevice));
   }
-  p->tw[p->ntw++] = {L, dptr};
+  p->tw[p->ntw++] = {L, rp, dptr};
   return 0;
 }
 
@@ -229,10 +234,10 @@ template <typename T> int row(const pf_plan* p, int L, int mode, int v, const Ro
     LaunchLog& g = *p->rec;
     if (g.n >= 8) { g.overflow = true; return 0; }
     LaunchRec& r = g.r[g.n++];
-    r.row = true; r.L = L; r.kind = mode; r.v = v; r.ra = a; r.tw = tw_of(p, L);
+    r.row = true; r.L = L; r.kind = mode; r.v = v; r.ra = a; r.tw = tw_of(p, L, row_radix_plan(v));
     return 0;
   }
-  return launch_row<T>(L, mode, a, tw_of(p, L), s, v);
+  return launch_row<T>(L, mode, a, tw_of(p, L, row_radix_plan(v)), s, v);
 }
 
 // A column pass launch, or its record while the plan records.
@@ -241,10 +246,10 @@ template <typename T> int emit_col(pf_plan* p, int L, int kind, const ColArgs& a
     LaunchLog& g = *p->rec;
     if (g.n >= 8) { g.overflow = true; return 0; }
     LaunchRec& r = g.r[g.n++];
-    r.row = false; r.L = L; r.kind = kind; r.v = v; r.ca = a; r.tw = tw_of(p, L);
+    r.row = false; r.L = L; r.kind = kind; r.v = v; r.ca = a; r.tw = tw_of(p, L, col_radix_plan(v));
     return 0;
   }
-  return launch_col<T>(L, kind, a, tw_of(p, L), s, v);
+  return launch_col<T>(L, kind, a, tw_of(p, L, col_radix_plan(v)), s, v);
 }
 
 ColArgs col_args() {
@@ -1538,7 +1543,7 @@ int pf_rhs_nonlinear_noalloc(const pf_desc* desc, const void* u_hat, void* rhs, 
   char* b = (char*)base;
   for (int t = 0; t < nl; ++t) {
     if ((size_t)tw_size(lens[t]) * p.csz > kTwSlot) return failf(PF_EINVAL, "twiddle table of length %d too large", lens[t]);
-    p.tw[p.ntw++] = {lens[t], b + (size_t)t * kTwSlot};
+    p.tw[p.ntw++] = {lens[t], 0, b + (size_t)t * kTwSlot};
   }
   p.w1 = b + tw_bytes;
   p.w2 = w2c ? b + tw_bytes + up(p.w1_bytes) : nullptr;

@@ -19,11 +19,13 @@ struct RadixPlan {
   int r[6];
 };
 
-constexpr RadixPlan make_plan(int L) {
-  // Lengths with a factor 3 whose radix-2/3/4/8/16 plan needs three or more
-  // passes: the mixed radices 6, 9, 12 and 24 bring them to two (three for
-  // 1536), one LDS round and barrier pair fewer per transform.
-  switch (L) {
+// Radix plan rp of length L.  rp = 0, the default: lengths with a factor 3
+// whose radix-2/3/4/8/16 plan needs three or more passes use the mixed
+// radices 6, 9, 12 and 24 to get by with two (three for 1536), one LDS round
+// and barrier pair fewer per transform.  rp = 1: the radix-2/3/4/8/16 plan
+// (radix 3 first), an alternative the autotuner times where it differs.
+constexpr RadixPlan make_plan(int L, int rp = 0) {
+  if (rp == 0) switch (L) {
     case 18: return RadixPlan{2, {9, 2, 0, 0, 0, 0}};
     case 36: return RadixPlan{2, {9, 4, 0, 0, 0, 0}};
     case 72: return RadixPlan{2, {9, 8, 0, 0, 0, 0}};
@@ -56,6 +58,9 @@ constexpr RadixPlan make_plan(int L) {
   }
   return p;
 }
+
+// whether radix plan 1 of L differs from plan 0
+constexpr bool has_alt_plan(int L) { return make_plan(L, 0).n != make_plan(L, 1).n; }
 
 constexpr bool plan_ok(int L) {
   int n = L;
@@ -224,8 +229,8 @@ template <int R, int S, typename T> __device__ __forceinline__ void dft(cplx<T>*
 // Twiddle table, one block per pass p >= 1 (pass 0 has span 1 and none):
 // entry OFF(p) + (r - 1) Ns + k = exp(-2 pi i r k / (Ns R)).  Lanes that
 // take consecutive butterflies read consecutive entries.
-constexpr int tw_offset(int L, int P) {
-  RadixPlan pl = make_plan(L);
+constexpr int tw_offset(int L, int P, int rp = 0) {
+  RadixPlan pl = make_plan(L, rp);
   int off = 0, ns = 1;
   for (int q = 0; q < P; ++q) {
     if (q > 0) off += (pl.r[q] - 1) * ns;
@@ -233,8 +238,8 @@ constexpr int tw_offset(int L, int P) {
   }
   return off;
 }
-constexpr int tw_size(int L) { return tw_offset(L, make_plan(L).n); }
-constexpr int tw_alloc(int L) { return tw_size(L) > 0 ? tw_size(L) : 1; }
+constexpr int tw_size(int L, int rp = 0) { return tw_offset(L, make_plan(L, rp).n, rp); }
+constexpr int tw_alloc(int L, int rp = 0) { return tw_size(L, rp) > 0 ? tw_size(L, rp) : 1; }
 
 // conjugated for the inverse
 template <int S, typename T> __device__ __forceinline__ cplx<T> twiddle(const cplx<T>* tw, int m) {
@@ -248,9 +253,10 @@ template <int S, typename T> __device__ __forceinline__ cplx<T> twiddle(const cp
 // consecutive threads to consecutive transforms (column tiles); otherwise
 // to consecutive butterflies of one transform (contiguous rows).
 // PEND: stop before pass PEND (the caller runs the remaining passes).
-template <typename T, int L, int S, int NF, int NT, int P, int NS, bool FFAST, class Addr, int PEND = 99>
+// RP: the radix plan (make_plan).
+template <typename T, int L, int S, int NF, int NT, int P, int NS, bool FFAST, class Addr, int PEND = 99, int RP = 0>
 __device__ __forceinline__ void stockham(cplx<T>* buf, const cplx<T>* tw, const Addr& addr) {
-  constexpr RadixPlan pl = make_plan(L);
+  constexpr RadixPlan pl = make_plan(L, RP);
   if constexpr (P < pl.n && P < PEND) {
     constexpr int R = pl.r[P];
     constexpr int NB = L / R;
@@ -267,7 +273,7 @@ __device__ __forceinline__ void stockham(cplx<T>* buf, const cplx<T>* tw, const 
 #pragma unroll
         for (int r = 0; r < R; ++r) v[kk][r] = buf[addr(f, j + r * NB)];
         if constexpr (NS > 1) {
-          constexpr int OFF = tw_offset(L, P);
+          constexpr int OFF = tw_offset(L, P, RP);
 #pragma unroll
           for (int r = 1; r < R; ++r) v[kk][r] = mul<T>(v[kk][r], twiddle<S, T>(tw, OFF + (r - 1) * NS + k));
         }
@@ -288,24 +294,24 @@ __device__ __forceinline__ void stockham(cplx<T>* buf, const cplx<T>* tw, const 
       }
     }
     __syncthreads();
-    stockham<T, L, S, NF, NT, P + 1, NS * R, FFAST, Addr, PEND>(buf, tw, addr);
+    stockham<T, L, S, NF, NT, P + 1, NS * R, FFAST, Addr, PEND, RP>(buf, tw, addr);
   }
 }
 
 // NF independent length-L transforms held in LDS.  Callers synchronise the
 // block before the call; the call ends with a block barrier.
-template <typename T, int L, int S, int NF, int NT, bool FFAST, class Addr>
+template <typename T, int L, int S, int NF, int NT, bool FFAST, int RP = 0, class Addr>
 __device__ __forceinline__ void lds_fft(cplx<T>* buf, const cplx<T>* tw, const Addr& addr) {
   static_assert(plan_ok(L), "FFT length must be 2^a 3^b with 1 <= a <= 11");
-  stockham<T, L, S, NF, NT, 0, 1, FFAST, Addr>(buf, tw, addr);
+  stockham<T, L, S, NF, NT, 0, 1, FFAST, Addr, 99, RP>(buf, tw, addr);
 }
 
 // All passes but the last (the caller runs it); ends with a block barrier
 // when any pass ran.
-template <typename T, int L, int S, int NF, int NT, class Addr>
+template <typename T, int L, int S, int NF, int NT, int RP = 0, class Addr>
 __device__ __forceinline__ void lds_fft_head(cplx<T>* buf, const cplx<T>* tw, const Addr& addr) {
   static_assert(plan_ok(L), "FFT length must be 2^a 3^b with 1 <= a <= 11");
-  stockham<T, L, S, NF, NT, 0, 1, true, Addr, make_plan(L).n - 1>(buf, tw, addr);
+  stockham<T, L, S, NF, NT, 0, 1, true, Addr, make_plan(L, RP).n - 1, RP>(buf, tw, addr);
 }
 
 }  // namespace pf

@@ -73,10 +73,12 @@ template <typename T> __device__ __forceinline__ cplx<T> pack(cplx<T> x, cplx<T>
 }
 
 // Row-pair variants of a row pass (the autotuner's choices): V = 0 the
-// default count, 1 half of it, 2 twice it; 0 where a variant repeats another
-// or does not fit in LDS.
+// default count, 1 half of it, 2 twice it, 3 the default count with radix
+// plan 1 (make_plan) where L has one; 0 where a variant repeats another or
+// does not fit in LDS.
 template <typename T, int MODE> constexpr int row_pairs_v(int L, int V) {
   const int np = row_pairs<T, MODE>(L);
+  if (V == 3) return has_alt_plan(L) ? np : 0;
   const int r = V == 0 ? np : (V == 1 ? np / 2 : np * 2);
   if (r < 1 || (V != 0 && r == np)) return 0;
   const long long lds = ((long long)r * row_slots<MODE>() * row_pitch(L) + L) * (long long)sizeof(cplx<T>);
@@ -84,21 +86,21 @@ template <typename T, int MODE> constexpr int row_pairs_v(int L, int V) {
 }
 
 // LDS of a row pass block: the twiddle table, then NP pairs of NS slots
-template <typename T, int L, int MODE, int NP> constexpr int row_lds_tw() { return tw_alloc(L); }
+template <typename T, int L, int MODE, int NP, int RP = 0> constexpr int row_lds_tw() { return tw_alloc(L, RP); }
 template <typename T, int L, int MODE, int NP> constexpr int row_lds_buf() {
   return NP * row_slots<MODE>() * row_pitch(L);
 }
 
 // One block of a row pass: block (bx, by) of its launch; tws and buf are the
 // block's LDS (row_lds_tw and row_lds_buf entries).
-template <typename T, int L, int MODE, int NP>
+template <typename T, int L, int MODE, int NP, int RP = 0>
 __device__ __forceinline__ void row_body(const RowArgs& a, const cplx<T>* __restrict__ tw, cplx<T>* tws,
                                          cplx<T>* buf, int bx, int by) {
   constexpr int NS = row_slots<MODE>();
   constexpr int LP = row_pitch(L);
   // the table size as a constant (evaluated at run time, the plan's radix
   // array would live in private memory)
-  constexpr int TWN = tw_size(L);
+  constexpr int TWN = tw_size(L, RP);
   for (int m = threadIdx.x; m < TWN; m += kThreads) tws[m] = tw[m];
 
   const int npairs = a.n_rows / 2;
@@ -202,9 +204,9 @@ __device__ __forceinline__ void row_body(const RowArgs& a, const cplx<T>* __rest
   __syncthreads();
 
   if constexpr (MODE == ROW_R2C) {
-    lds_fft<T, L, -1, NP, kThreads, false>(buf, tws, ix);
+    lds_fft<T, L, -1, NP, kThreads, false, RP>(buf, tws, ix);
   } else {
-    lds_fft<T, L, +1, NP * NS, kThreads, false>(buf, tws, ix);
+    lds_fft<T, L, +1, NP * NS, kThreads, false, RP>(buf, tws, ix);
   }
 
   if constexpr (MODE == ROW_C2R) {
@@ -247,7 +249,7 @@ __device__ __forceinline__ void row_body(const RowArgs& a, const cplx<T>* __rest
       const int slot = p * 6 + (w == 0 ? 0 : (w == 1 ? 3 : 1));
       return slot * LP + e + (e >> 4);
     };
-    lds_fft<T, L, -1, NP * 3, kThreads, false>(buf, tws, fx);
+    lds_fft<T, L, -1, NP * 3, kThreads, false, RP>(buf, tws, fx);
   } else if constexpr (MODE == ROW_FUSED2) {
     for (int t = threadIdx.x; t < NP * L; t += kThreads) {
       const int p = t / L, x = t % L;
@@ -263,7 +265,7 @@ __device__ __forceinline__ void row_body(const RowArgs& a, const cplx<T>* __rest
       const int p = ff / 2, w = ff % 2;
       return (p * 3 + 2 * w) * LP + e + (e >> 4);
     };
-    lds_fft<T, L, -1, NP * 2, kThreads, false>(buf, tws, fx);
+    lds_fft<T, L, -1, NP * 2, kThreads, false, RP>(buf, tws, fx);
   }
 
   // ---- split the packed spectra, truncate, scale, store.  TPO threads own
@@ -313,11 +315,11 @@ __device__ __forceinline__ void row_body(const RowArgs& a, const cplx<T>* __rest
   }
 }
 
-template <typename T, int L, int MODE, int NP>
+template <typename T, int L, int MODE, int NP, int RP = 0>
 __global__ void __launch_bounds__(kThreads) row_pass(RowArgs a, const cplx<T>* __restrict__ tw) {
-  __shared__ cplx<T> tws[row_lds_tw<T, L, MODE, NP>()];
+  __shared__ cplx<T> tws[row_lds_tw<T, L, MODE, NP, RP>()];
   __shared__ cplx<T> buf[row_lds_buf<T, L, MODE, NP>()];
-  row_body<T, L, MODE, NP>(a, tw, tws, buf, blockIdx.x, blockIdx.y);
+  row_body<T, L, MODE, NP, RP>(a, tw, tws, buf, blockIdx.x, blockIdx.y);
 }
 
 // -------------------------------------------------------------- random numbers
@@ -1051,11 +1053,13 @@ template <typename T, int KIND> constexpr int col_pipe_v(int L, int V) {
 }
 // Launch variants (the autotuner's choices): V = 0 the default tile width,
 // 1 half of it, 2 twice it, 3 the default width with two tiles per block,
-// 4 (3D stage kind) twice the width with 1024-thread blocks; width 0 where a
+// 4 (3D stage kind) twice the width with 1024-thread blocks, 5 the default
+// width with radix plan 1 (make_plan) where L has one; width 0 where a
 // variant does not fit in LDS, does not exist or would repeat variant 0.
 template <typename T, int KIND> constexpr int col_width_v(int L, int V) {
   const int base = col_width<T, KIND>(L);
   if (base == 0 || V == 0) return base;
+  if (V == 5) return has_alt_plan(L) ? base : 0;
   if (V == 3) return col_pipe_v<T, KIND>(L, 3) == 2 ? base : 0;
   // 1024-thread blocks cap a thread at 128 registers: the 2D stage update
   // would spill there, so variant 4 is the 3D stage kind's only
@@ -1098,12 +1102,12 @@ template <int KIND> constexpr int col_fields() {
   return KIND == K_FWD_STAGE3 ? 3 : (KIND == K_FWD_STAGE2 ? 2 : 1);
 }
 // LDS of a column pass block: the twiddle table, then L rows of the tile
-template <typename T, int L, int TC, int KIND> constexpr int col_lds_tw() { return tw_alloc(L); }
+template <typename T, int L, int TC, int KIND, int RP = 0> constexpr int col_lds_tw() { return tw_alloc(L, RP); }
 template <typename T, int L, int TC, int KIND> constexpr int col_lds_buf() { return L * col_fields<KIND>() * TC; }
 
 // One block of a column pass: block bid of the nblk blocks of its launch;
 // tws and buf are the block's LDS (col_lds_tw and col_lds_buf entries).
-template <typename T, int L, int TC, int KIND, int PIPE, int NT>
+template <typename T, int L, int TC, int KIND, int PIPE, int NT, int RP = 0>
 __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __restrict__ tw, cplx<T>* tws,
                                          cplx<T>* buf, long long bid, long long nblk) {
   constexpr bool STAGE = KIND == K_FWD_STAGE3 || KIND == K_FWD_STAGE2;
@@ -1116,7 +1120,7 @@ __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __rest
   constexpr int NIT = (L * W + NT - 1) / NT;
   constexpr int CH = NIT < 16 ? NIT : 16;  // loads kept in flight per thread
   static_assert(NT % TC == 0, "tile width must divide the block size");
-  constexpr int TWN = tw_size(L);  // a constant, as in row_pass
+  constexpr int TWN = tw_size(L, RP);  // a constant, as in row_pass
   for (int m = threadIdx.x; m < TWN; m += NT) tws[m] = tw[m];
 
   const int nfg = STAGE ? 1 : a.n_fields;
@@ -1276,7 +1280,7 @@ __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __rest
   // direct form measured 3-7% slower at lengths 192 and 144, whose first
   // radix is 3), and so does an inverse whose plan has a single pass (its
   // last pass stores on its own).
-  constexpr RadixPlan plD = make_plan(L);
+  constexpr RadixPlan plD = make_plan(L, RP);
   constexpr bool DIRECT =
       PIPE == 1 && (KIND == K_FWD || ((KIND == K_INV || UC) && plD.n >= 2));
   if constexpr (!DIRECT) {
@@ -1399,15 +1403,15 @@ __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __rest
     __syncthreads();
     if constexpr (INV) {
       // the middle passes; the last one stores to global memory below
-      stockham<T, L, +1, W, NT, 1, R0, true, decltype(addr), plD.n - 1>(buf, tws, addr);
+      stockham<T, L, +1, W, NT, 1, R0, true, decltype(addr), plD.n - 1, RP>(buf, tws, addr);
     } else {
-      stockham<T, L, -1, W, NT, 1, R0, true>(buf, tws, addr);
+      stockham<T, L, -1, W, NT, 1, R0, true, decltype(addr), 99, RP>(buf, tws, addr);
     }
   } else if constexpr (INV) {
     // every pass but the last in LDS; the last one stores to global memory
-    lds_fft_head<T, L, +1, W, NT>(buf, tws, addr);
+    lds_fft_head<T, L, +1, W, NT, RP>(buf, tws, addr);
   } else {
-    lds_fft<T, L, -1, W, NT, true>(buf, tws, addr);
+    lds_fft<T, L, -1, W, NT, true, RP>(buf, tws, addr);
   }
 
   if constexpr (INV) {
@@ -1417,13 +1421,13 @@ __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __rest
     // fewer than staging the transformed tile.
     cplx<T>* out = (cplx<T>*)a.out + (long long)b * a.out_member + (long long)(a.f_base + fg) * a.out_field +
                    (long long)outer * gm.out_outer;
-    constexpr RadixPlan plI = make_plan(L);
+    constexpr RadixPlan plI = make_plan(L, RP);
     constexpr int PL = plI.n - 1;
     constexpr int RL = plI.r[PL];
     constexpr int NSL = L / RL;  // the last pass's span = its butterflies per transform
     constexpr int TOTL = W * NSL;
     constexpr int KL = (TOTL + NT - 1) / NT;
-    constexpr int OFFL = tw_offset(L, PL);
+    constexpr int OFFL = tw_offset(L, PL, RP);
     // W = TC divides the block size: every butterfly of a thread is in its column
     const Slot os = slot(threadIdx.x % TC);
     const long long ocol = (long long)os.j * gm.out_lp + os.l;
@@ -1518,11 +1522,11 @@ __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __rest
 }
 
 
-template <typename T, int L, int TC, int KIND, int PIPE, int NT>
+template <typename T, int L, int TC, int KIND, int PIPE, int NT, int RP = 0>
 __global__ void __launch_bounds__(NT) col_pass(ColArgs a, const cplx<T>* __restrict__ tw) {
-  __shared__ cplx<T> tws[col_lds_tw<T, L, TC, KIND>()];
+  __shared__ cplx<T> tws[col_lds_tw<T, L, TC, KIND, RP>()];
   __shared__ cplx<T> buf[col_lds_buf<T, L, TC, KIND>()];
-  col_body<T, L, TC, KIND, PIPE, NT>(a, tw, tws, buf, blockIdx.x, gridDim.x);
+  col_body<T, L, TC, KIND, PIPE, NT, RP>(a, tw, tws, buf, blockIdx.x, gridDim.x);
 }
 
 

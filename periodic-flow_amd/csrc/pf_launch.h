@@ -14,10 +14,14 @@ bool length_supported(int L);
 
 // Launch variants: V = 0 is the default tile (column passes) or row-pair
 // count (row passes); FP32 also builds V = 1 (half), V = 2 (twice) and, for
-// column passes, V = 3 (two tiles per block, the second prefetched) and
-// V = 4 (stage kinds: 1024-thread blocks of twice the width), the choices
-// the autotuner times.  One translation unit per precision and V.
-constexpr int kVariants = 5;
+// column passes, V = 3 (two tiles per block, the second prefetched), V = 4
+// (stage kinds: 1024-thread blocks of twice the width) and V = 5 (radix
+// plan 1); for row passes V = 3 is radix plan 1.  These are the choices the
+// autotuner times.  One translation unit per precision and V.
+constexpr int kVariants = 6;
+// the radix plan (make_plan) a launch variant uses
+constexpr int col_radix_plan(int v) { return v == 5 ? 1 : 0; }
+constexpr int row_radix_plan(int v) { return v == 3 ? 1 : 0; }
 template <typename T, int V> int launch_col_v(int L, int kind, const ColArgs& a, const void* tw, hipStream_t s);
 template <typename T, int V> int col_width_q(int L, int kind);
 template <typename T, int V> int launch_row_v(int L, int mode, const RowArgs& a, const void* tw, hipStream_t s);
@@ -35,6 +39,7 @@ int launch_col(int L, int kind, const ColArgs& a, const void* tw, hipStream_t s,
       case 2: return launch_col_v<T, 2>(L, kind, a, tw, s);
       case 3: return launch_col_v<T, 3>(L, kind, a, tw, s);
       case 4: return launch_col_v<T, 4>(L, kind, a, tw, s);
+      case 5: return launch_col_v<T, 5>(L, kind, a, tw, s);
       default: return -1;
     }
   } else {
@@ -50,6 +55,7 @@ template <typename T> int tile_cols(int L, int kind, int v) {
       case 2: return col_width_q<T, 2>(L, kind);
       case 3: return col_width_q<T, 3>(L, kind);
       case 4: return col_width_q<T, 4>(L, kind);
+      case 5: return col_width_q<T, 5>(L, kind);
       default: return 0;
     }
   } else {
@@ -64,8 +70,9 @@ int launch_row(int L, int mode, const RowArgs& a, const void* tw, hipStream_t s,
       case 0: return launch_row_v<T, 0>(L, mode, a, tw, s);
       case 1: return launch_row_v<T, 1>(L, mode, a, tw, s);
       case 2: return launch_row_v<T, 2>(L, mode, a, tw, s);
-      case 3:
-      case 4: return -4;  // row passes have no such variants
+      case 3: return launch_row_v<T, 3>(L, mode, a, tw, s);
+      case 4:
+      case 5: return -4;  // row passes have no such variants
       default: return -1;
     }
   } else {
@@ -79,6 +86,7 @@ template <typename T> int row_pair_count(int L, int mode, int v) {
       case 0: return row_pairs_q<T, 0>(L, mode);
       case 1: return row_pairs_q<T, 1>(L, mode);
       case 2: return row_pairs_q<T, 2>(L, mode);
+      case 3: return row_pairs_q<T, 3>(L, mode);
       default: return 0;
     }
   } else {
