@@ -226,23 +226,110 @@ def serving_bench(torch, count=40, max_steps=300):
             "throughput_speedup": round(batched["Mpoint_steps_per_s"] / max(serial["Mpoint_steps_per_s"], 1e-9), 2)}
 
 
+def _note(*a):
+    print("[suite]", *a, file=sys.stderr, flush=True)
+
+
+def tgv64_vs_oracle(torch):
+    """Taylor-Green 64^3 (3/2 rule, RK3, FP64, 200 steps): energy and
+    dissipation every step on the GPU and in the oracle (scipy.fft over the
+    host's cores), and the final fields."""
+    import numpy as np
+
+    from oracle import pf_oracle as po
+    from periodicflow import workloads as W
+
+    po.set_fft_workers(len(os.sched_getaffinity(0)))
+    r = W.tgv64_rk3(200)
+    o = po.OracleSolver([64] * 3, 0.01, time_integrator="rk3", viscosity=1 / 400, dealias="3/2")
+    o.initialize(po.taylor_green_3d(o.g))
+    E, eps = [po.energy(o.g, o.u_hat)], [po.dissipation(o.g, o.u_hat, 1 / 400)]
+    for _ in range(200):
+        o.solve(1, noise=[(0.0, 0.0)])
+        E.append(po.energy(o.g, o.u_hat))
+        eps.append(po.dissipation(o.g, o.u_hat, 1 / 400))
+    E, eps = np.array(E), np.array(eps)
+    return {"E0": float(r["energy"][0]), "E200": float(r["energy"][-1]),
+            "eps0": float(r["dissipation"][0]), "eps200": float(r["dissipation"][-1]),
+            "E_max_rel_err_vs_oracle": float(np.max(np.abs(r["energy"] - E) / E)),
+            "eps_max_rel_err_vs_oracle": float(np.max(np.abs(r["dissipation"] - eps) / eps)),
+            "u_hat_rel_l2_vs_oracle": po.rel_l2(r["u_hat"].cpu().numpy()[0], o.u_hat),
+            "history_every_20_steps": {"E": [float(x) for x in r["energy"][::20]],
+                                       "eps": [float(x) for x in r["dissipation"][::20]]}}
+
+
+def forced_fp32_vs_fp64(torch, steps=1000, every=50):
+    """The forced-turbulence configuration's FP32 error against FP64 over its
+    1000 steps: one 128^3 and one 96^3 member (seeds 2000 and 2100, on-device
+    noise identical in both precisions), relative L2 of u_hat every ``every``
+    steps.  Chaos amplifies rounding, so the error grows with the steps."""
+    from periodicflow import workloads as W
+    from periodicflow.ensemble import random_solenoidal
+    from periodicflow.plan import Plan
+
+    out = {}
+    for spec, seed in (W.forced_members(1, 1)):
+        plans, states = {}, {}
+        for prec in ("fp32", "fp64"):
+            p = Plan(spec.n, spec.dt, precision=prec, integrator="rk3", viscosity=spec.viscosity,
+                     noise_type="correlated", noise_mag=spec.noise_mag, forcing="constant_power",
+                     forcing_power=spec.forcing_power, forcing_band=spec.forcing_band, seeds=[seed])
+            p.set_correlation(torch.as_tensor(W.forced_correlation(spec), device="cuda", dtype=p.complex_dtype))
+            u = torch.as_tensor(random_solenoidal(p, seed, energy=0.5, kp=4.0)[None], device="cuda",
+                                dtype=p.complex_dtype).contiguous()
+            plans[prec], states[prec] = p, u
+        curve = []
+        for done in range(0, steps + 1, every):
+            if done:
+                for prec in ("fp32", "fp64"):
+                    plans[prec].step(states[prec], every)
+            a, b = states["fp32"].to(torch.complex128), states["fp64"]
+            curve.append([done, float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))])
+        out[str(spec.n[0])] = curve
+        _note("forced fp32 vs fp64", spec.n, curve[-1])
+    return out
+
+
+def tgv256_fp64_rate(torch, steps=50):
+    """The plain Taylor-Green 256^3 workload in FP64 (validation mode)."""
+    from periodicflow.plan import Plan
+
+    p = Plan([N] * 3, DT, precision="fp64", integrator="rk4", viscosity=NU, dealias="2/3")
+    u = _tgv_state(p, torch)
+    p.step(u, 2)
+    r = _rate(torch, lambda: p.step(u, steps), N**3 * steps)
+    return {"steps": steps, "Mpoint_steps_per_s": round(r, 1)}
+
+
 def suite(torch, path):
     """The long configurations, written to ``path`` (one JSON object)."""
     from periodicflow import workloads as W
     from periodicflow.serving import Engine, GpuBackend, serve_one_at_a_time
 
+    import logging
+
+    logging.basicConfig(level=logging.INFO, stream=sys.stderr, format="[%(name)s] %(message)s")
     res = {}
-    r = W.tgv64_rk3(200)
-    res["tgv64_rk3_fp64"] = {"E0": float(r["energy"][0]), "E200": float(r["energy"][-1]),
-                             "eps0": float(r["dissipation"][0]), "eps200": float(r["dissipation"][-1])}
+    _note("tgv64 vs oracle")
+    res["tgv64_rk3_fp64"] = tgv64_vs_oracle(torch)
+    _note("tgv256 fp64")
+    res["tgv256_rk4_fp64"] = tgv256_fp64_rate(torch)
+    _note("stokes512")
     res["stokes512_thermal"] = W.stokes512()
+    _note("forced3d")
     f = W.forced3d(steps=100)
     f.pop("ensemble")
     res["forced3d_16x128_16x96"] = f
-    reqs = W.serving_requests(count=60, max_steps=400)
-    res["serving_batched"] = Engine(GpuBackend("fp32"), hbm_budget=96 << 30, keep_results=False).run(reqs)
-    reqs = W.serving_requests(count=60, max_steps=400)
-    res["serving_one_at_a_time"] = serve_one_at_a_time(GpuBackend("fp32"), reqs, hbm_budget=96 << 30)[0]
+    res["forced3d_fp32_vs_fp64"] = forced_fp32_vs_fp64(torch)
+    for count, cap in ((400, 2000),):
+        _note("serving", count, cap)
+        reqs = W.serving_requests(count=count, max_steps=cap)
+        b = Engine(GpuBackend("fp32"), hbm_budget=96 << 30, keep_results=False, log_every=20).run(reqs)
+        _note("serving one at a time")
+        reqs = W.serving_requests(count=count, max_steps=cap)
+        one = serve_one_at_a_time(GpuBackend("fp32"), reqs, hbm_budget=96 << 30, keep_results=False,
+                                  log_every=20)[0]
+        res[f"serving_{count}x{cap}"] = {"batched": b, "one_at_a_time": one}
     with open(path, "w") as fh:
         json.dump(res, fh, indent=1, default=float)
     return res

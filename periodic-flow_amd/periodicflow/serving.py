@@ -40,6 +40,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
+from . import logger
 from .ensemble import MemberSpec
 
 
@@ -187,7 +188,8 @@ class Engine:
     'all' (see the module docstring)."""
 
     def __init__(self, backend, hbm_budget=64 << 30, chunk=200, max_slots=64, keep_results=True,
-                 clock=time.perf_counter, policy="srpt", round_s=0.05, check_every=8, overlap=1.25):
+                 clock=time.perf_counter, policy="srpt", round_s=0.05, check_every=8, overlap=1.25,
+                 log_every=0):
         self.be = backend
         self.budget = int(hbm_budget)
         self.chunk = int(chunk)
@@ -203,6 +205,7 @@ class Engine:
         self.round_s = float(round_s)
         self.check_every = max(1, int(check_every))
         self.overlap = float(overlap)
+        self.log_every = int(log_every)  # report every n-th finished request (0: never)
         if policy not in ("srpt", "all"):
             raise ValueError("policy must be 'srpt' or 'all'")
         self.policy = policy
@@ -292,6 +295,8 @@ class Engine:
         r.t_done = self._now()
         self.finished.append(r)
         pool.last_used = r.t_done
+        if self.log_every and len(self.finished) % self.log_every == 0:
+            logger.info(f"serving: {len(self.finished)} requests finished at {r.t_done:.1f} s")
 
     def _remaining(self, pool):
         """Work left before the pool's shortest member finishes: estimated
@@ -403,9 +408,10 @@ class Engine:
         }
 
 
-def serve_one_at_a_time(backend, requests, hbm_budget=64 << 30, chunk=1 << 20, keep_results=True):
+def serve_one_at_a_time(backend, requests, hbm_budget=64 << 30, chunk=1 << 20, keep_results=True, log_every=0):
     """The plain baseline: each request alone (batch 1) in arrival order,
     its whole step count in one call, through the same engine; latency
     includes queueing."""
-    eng = Engine(backend, hbm_budget=hbm_budget, chunk=chunk, max_slots=1, round_s=1e9, keep_results=keep_results)
+    eng = Engine(backend, hbm_budget=hbm_budget, chunk=chunk, max_slots=1, round_s=1e9, keep_results=keep_results,
+                 log_every=log_every)
     return eng.run(requests, serial=True), eng.finished
