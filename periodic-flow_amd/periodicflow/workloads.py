@@ -272,3 +272,64 @@ def solenoidal_device(plan, seed, energy=0.5, kp=4.0):
     if e > 0:
         u = u * math.sqrt(energy / e)
     return u.to(plan.complex_dtype)
+
+
+def _calibration_copy():
+    """One 1 GiB device-to-device copy: the kernel whose counters scale the
+    HBM traffic counters of a profile (known bytes read and written)."""
+    src = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+
+
+def main(argv=None):
+    """Run one configuration for a few steps (the target of kernel profiles):
+
+    python -m periodicflow.workloads stokes512 --steps 600
+        thermal-noise Stokes 512^2 x 256 realisations, register-resident
+        steps with the FP64 structure factor accumulated
+    python -m periodicflow.workloads forced --steps 5
+        the forced-turbulence ensemble 16 x 128^3 + 16 x 96^3, default plans
+        (no autotuning, so the trace holds only the steps)
+    python -m periodicflow.workloads tgv256 --steps 5
+        Taylor-Green 256^3, RK4, 2/3 rule, FP32
+
+    --calibrate-copy adds one 1 GiB device copy (__amd_rocclr_copyBuffer).
+    """
+    import argparse
+
+    ap = argparse.ArgumentParser(prog="python -m periodicflow.workloads")
+    ap.add_argument("workload", choices=["stokes512", "forced", "tgv256"])
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--calibrate-copy", action="store_true")
+    args = ap.parse_args(argv)
+    if args.workload == "stokes512":
+        n, nu, dt, kT, B = 512, 1.0, 0.05, 1e-3, 256
+        nm = math.sqrt(2 * nu * kT / (2 * math.pi / n) ** 2)
+        p = Plan((n, n), dt, precision="fp32", integrator="implicit_pc", nonlinear=False, viscosity=nu,
+                 noise_mag=nm, batch=B, seeds=[1000 + r for r in range(B)])
+        u = p.new_state()
+        acc = torch.zeros(p.shape_fourier, dtype=torch.float64, device="cuda")
+        t = _timed(lambda: p.step_accumulate_sk(u, args.steps, acc))
+        rate = n * n * B * args.steps / t / 1e6
+    elif args.workload == "forced":
+        members = forced_members()
+        ens = Ensemble(members, precision="fp32", correlation=forced_correlation)
+        for i, (spec, seed) in enumerate(members):
+            ens.set_fourier(i, random_solenoidal(ens.plan_of(i), seed, energy=0.5, kp=4.0))
+        t = _timed(lambda: ens.step(args.steps))
+        rate = ens.points_per_step() * args.steps / t / 1e6
+    else:
+        p = Plan([256] * 3, 2.5e-3, precision="fp32", integrator="rk4", viscosity=1 / 1600, dealias="2/3")
+        u = tgv_state(p)
+        t = _timed(lambda: p.step(u, args.steps))
+        rate = 256**3 * args.steps / t / 1e6
+    if args.calibrate_copy:
+        _calibration_copy()
+    print(f"{args.workload}: {args.steps} steps in {t:.3f} s, {rate:.1f} Mpoint-steps/s", flush=True)
+
+
+if __name__ == "__main__":
+    main()
