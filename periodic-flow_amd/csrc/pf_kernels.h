@@ -48,6 +48,15 @@ template <typename T, int MODE> constexpr int row_pairs(int L) {
   return p < 1 ? 1 : (p > 8 ? 8 : p);
 }
 __host__ __device__ constexpr int row_pitch(int L) { return L + L / 16; }
+// Row entry e of a slot sits at e + e / row_pad_div(L): one skipped entry
+// every 16 for power-of-two lengths (the radix-16/8 strides then fall on
+// distinct banks), every 48 (96 at length 144) where L has a factor 3, whose
+// radix-9/12/24 first pass writes with stride 9, 12 or 24.  Chosen by
+// counting bank conflicts of every LDS access of the row pass (64 lanes in
+// groups of 16 over 32 banks and of 32 over 64); at lengths 96, 192, 288 and
+// 384 this halves them or better, e.g. 2.4 to 0.7 conflict cycles per
+// access at 192.
+__host__ __device__ constexpr int row_pad_div(int L) { return L % 3 ? 16 : (L == 144 ? 96 : 48); }
 
 // Hermitian extension of a stored half spectrum to index k of a length-L
 // complex transform.  DC keeps its real part only; the stored Nyquist entry
@@ -108,7 +117,9 @@ __device__ __forceinline__ void row_body(const RowArgs& a, const cplx<T>* __rest
   int f = 0, b;
   if (MODE == ROW_C2R || MODE == ROW_R2C) { f = by % a.n_fields; b = by / a.n_fields; }
   else b = by;
-  auto ix = [](int slot, int e) { return slot * LP + e + (e >> 4); };
+  constexpr int PD = row_pad_div(L);
+  static_assert(L - 1 + (L - 1) / PD < LP, "row pitch too small for the padding");
+  auto ix = [](int slot, int e) { return slot * LP + e + e / PD; };
   const int N = a.N;
 
   // ---- load (every load of a chunk in flight before the LDS stores)
@@ -247,7 +258,7 @@ __device__ __forceinline__ void row_body(const RowArgs& a, const cplx<T>* __rest
     auto fx = [](int ff, int e) {
       const int p = ff / 3, w = ff % 3;
       const int slot = p * 6 + (w == 0 ? 0 : (w == 1 ? 3 : 1));
-      return slot * LP + e + (e >> 4);
+      return slot * LP + e + e / PD;
     };
     lds_fft<T, L, -1, NP * 3, kThreads, false, RP>(buf, tws, fx);
   } else if constexpr (MODE == ROW_FUSED2) {
@@ -263,7 +274,7 @@ __device__ __forceinline__ void row_body(const RowArgs& a, const cplx<T>* __rest
     __syncthreads();
     auto fx = [](int ff, int e) {
       const int p = ff / 2, w = ff % 2;
-      return (p * 3 + 2 * w) * LP + e + (e >> 4);
+      return (p * 3 + 2 * w) * LP + e + e / PD;
     };
     lds_fft<T, L, -1, NP * 2, kThreads, false, RP>(buf, tws, fx);
   }
