@@ -105,7 +105,9 @@ class Ensemble:
 
     def autotune(self, reps=5):
         """Choose every group's launch plans by timing (Plan.autotune); returns
-        {group key: {pass name: chosen ms}}."""
+        {group key: {pass name: chosen ms}}.  A member then equals the same
+        member run alone bit for bit only under the same plan (the radix
+        factorisation changes rounding)."""
         self._tune = False
         self._graph = None
         return {k: g.plan.autotune(g.state, reps) for k, g in self.groups.items()}

@@ -278,7 +278,9 @@ class Plan:
     def autotune(self, u_hat, reps=5):
         """Time every launch variant of every nonlinear pass (and the fused
         against the unfused stage pass) on a copy of u_hat, keep the fastest;
-        returns {pass name: chosen average ms}."""
+        returns {pass name: chosen average ms}.  Every variant is bitwise the
+        default except the radix-2/3 factorisation of a length with a factor
+        3 (column variant 5, row variant 3), which changes rounding only."""
         self._check(u_hat, self.state_shape(), self.complex_dtype, "u_hat")
         n = len(_lib.PASS_NAMES)
         ms = (C.c_double * n)()
