@@ -1,8 +1,8 @@
 // Register butterflies and the LDS Stockham engine used by every FFT pass.
 //
-// A length-L transform (L = 2^a 3^b) runs as a sequence of radix passes
-// (make_plan: radix 3 first, then 16/8/4/2, or fewer passes with the mixed
-// radices 6, 9, 12, 24 where the length has a factor 3).  Pass p with radix R and span Ns (product
+// A length-L transform (L = 2^a 3^b 5^c) runs as a sequence of radix passes
+// (make_plan: radix 5 first, then 3, then 16/8/4/2, or fewer passes with the
+// mixed radices 6, 9, 12, 24 where the length is 2^a 3^b).  Pass p with radix R and span Ns (product
 // of the earlier radices) reads element j + r L/R, multiplies it by
 // W_{Ns R}^{r (j mod Ns)}, does a radix-R DFT in registers and writes it to
 // (j / Ns) Ns R + (j mod Ns) + r Ns: the Stockham auto-sort order, so the
@@ -38,9 +38,11 @@ constexpr RadixPlan make_plan(int L, int rp = 0) {
     default: break;
   }
   RadixPlan p{0, {0, 0, 0, 0, 0, 0}};
-  int a = 0, b = 0, n = L;
+  int a = 0, b = 0, c = 0, n = L;
   while (n % 2 == 0) { n /= 2; ++a; }
   while (n % 3 == 0) { n /= 3; ++b; }
+  while (n % 5 == 0) { n /= 5; ++c; }
+  for (int i = 0; i < c; ++i) p.r[p.n++] = 5;
   for (int i = 0; i < b; ++i) p.r[p.n++] = 3;
   switch (a) {
     case 1: p.r[p.n++] = 2; break;
@@ -66,9 +68,10 @@ constexpr bool plan_ok(int L) {
   int n = L;
   while (n % 2 == 0) n /= 2;
   while (n % 3 == 0) n /= 3;
+  while (n % 5 == 0) n /= 5;
   int a = 0;
   for (int m = L; m % 2 == 0; m /= 2) ++a;
-  return n == 1 && a >= 1 && a <= 11;
+  return n == 1 && a >= 1 && a <= 11 && make_plan(L).n <= 6;
 }
 
 // cos(2 pi m / 16) and sin(2 pi m / 16), m = 0..15
@@ -82,6 +85,9 @@ constexpr double kSin16[16] = {0.0, 0.38268343236508977173, 0.707106781186547524
                                -1.0, -0.92387953251128675613, -0.70710678118654752440, -0.38268343236508977173};
 constexpr double kSqrtHalf = 0.70710678118654752440;
 constexpr double kSqrt3Half = 0.86602540378443864676;
+// cos and sin of 2 pi / 5 and 4 pi / 5
+constexpr double kCos5a = 0.30901699437494742410, kCos5b = -0.80901699437494742410;
+constexpr double kSin5a = 0.95105651629515357212, kSin5b = 0.58778525229247312917;
 
 // multiply by W16^m = exp(S 2 pi i m / 16) for a compile-time m
 template <int S, int M, typename T> __device__ __forceinline__ cplx<T> w16(cplx<T> a) {
@@ -119,6 +125,24 @@ template <int S, typename T> __device__ __forceinline__ void dft3(cplx<T>* v) {
   v[0] = add<T>(v[0], t);
   v[1] = add<T>(m, d);
   v[2] = sub<T>(m, d);
+}
+
+// X_k = sum_n x_n w^(n k), w = exp(S 2 pi i / 5): the pairs (x1, x4) and
+// (x2, x3) enter X_k and X_(5-k) through their sums (real cosines) and
+// differences (imaginary sines)
+template <int S, typename T> __device__ __forceinline__ void dft5(cplx<T>* v) {
+  const cplx<T> t1 = add<T>(v[1], v[4]), t2 = add<T>(v[2], v[3]);
+  const cplx<T> t3 = sub<T>(v[1], v[4]), t4 = sub<T>(v[2], v[3]);
+  const T c1 = (T)kCos5a, c2 = (T)kCos5b, s1 = (T)kSin5a, s2 = (T)kSin5b;
+  const cplx<T> a1 = mk<T>(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
+  const cplx<T> a2 = mk<T>(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
+  const cplx<T> b1 = mul_si<S, T>(mk<T>(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y));
+  const cplx<T> b2 = mul_si<S, T>(mk<T>(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y));
+  v[0] = add<T>(v[0], add<T>(t1, t2));
+  v[1] = add<T>(a1, b1);
+  v[4] = sub<T>(a1, b1);
+  v[2] = add<T>(a2, b2);
+  v[3] = sub<T>(a2, b2);
 }
 
 template <int S, typename T> __device__ __forceinline__ void dft8(cplx<T>* v) {
@@ -216,6 +240,7 @@ template <int R1, int R2, int S, typename T> __device__ __forceinline__ void dft
 template <int R, int S, typename T> __device__ __forceinline__ void dft(cplx<T>* v) {
   if constexpr (R == 2) dft2<S, T>(v[0], v[1]);
   else if constexpr (R == 3) dft3<S, T>(v);
+  else if constexpr (R == 5) dft5<S, T>(v);
   else if constexpr (R == 4) dft4<S, T>(v[0], v[1], v[2], v[3]);
   else if constexpr (R == 8) dft8<S, T>(v);
   else if constexpr (R == 16) dft16<S, T>(v);
@@ -302,7 +327,7 @@ __device__ __forceinline__ void stockham(cplx<T>* buf, const cplx<T>* tw, const 
 // block before the call; the call ends with a block barrier.
 template <typename T, int L, int S, int NF, int NT, bool FFAST, int RP = 0, class Addr>
 __device__ __forceinline__ void lds_fft(cplx<T>* buf, const cplx<T>* tw, const Addr& addr) {
-  static_assert(plan_ok(L), "FFT length must be 2^a 3^b with 1 <= a <= 11");
+  static_assert(plan_ok(L), "FFT length must be 2^a 3^b 5^c with 1 <= a <= 11");
   stockham<T, L, S, NF, NT, 0, 1, FFAST, Addr, 99, RP>(buf, tw, addr);
 }
 
@@ -310,7 +335,7 @@ __device__ __forceinline__ void lds_fft(cplx<T>* buf, const cplx<T>* tw, const A
 // when any pass ran.
 template <typename T, int L, int S, int NF, int NT, int RP = 0, class Addr>
 __device__ __forceinline__ void lds_fft_head(cplx<T>* buf, const cplx<T>* tw, const Addr& addr) {
-  static_assert(plan_ok(L), "FFT length must be 2^a 3^b with 1 <= a <= 11");
+  static_assert(plan_ok(L), "FFT length must be 2^a 3^b 5^c with 1 <= a <= 11");
   stockham<T, L, S, NF, NT, 0, 1, true, Addr, make_plan(L, RP).n - 1, RP>(buf, tw, addr);
 }
 

@@ -2,11 +2,13 @@
 #pragma once
 #include "pf_kernels.h"
 
-// Transform lengths compiled into the library: 2^a 3^b covering N and 3N/2
-// for N in {8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 512, 1024}.
+// Transform lengths compiled into the library: N and 3N/2 for N in
+// {8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 512, 1024} and for N in
+// {20, 40, 80, 160, 320}, and N alone (the 2/3 rule) for N in {30, 60, 120,
+// 240, 480}.
 #define PF_LENGTHS(X) \
-  X(8) X(12) X(16) X(18) X(24) X(32) X(36) X(48) X(64) X(72) X(96) X(128) X(144) X(192) X(256) X(288) \
-  X(384) X(512) X(768) X(1024) X(1536)
+  X(8) X(12) X(16) X(18) X(20) X(24) X(30) X(32) X(36) X(40) X(48) X(60) X(64) X(72) X(80) X(96) X(120) \
+  X(128) X(144) X(160) X(192) X(240) X(256) X(288) X(320) X(384) X(480) X(512) X(768) X(1024) X(1536)
 
 namespace pf {
 

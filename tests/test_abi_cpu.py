@@ -74,8 +74,9 @@ def _desc(**kw):
     (dict(batch=0), -1, "batch"),
     (dict(dt=0.0), -1, "dt"),
     (dict(n=(15, 16, 16)), -1, "even"),
-    (dict(n=(20, 16, 16)), -4, "transform length"),
-    (dict(n=(16, 16, 40)), -4, "transform length"),
+    (dict(n=(14, 16, 16)), -4, "transform length"),   # factor 7
+    (dict(n=(16, 16, 44)), -4, "transform length"),   # factor 11
+    (dict(n=(16, 16, 30)), -4, "transform length"),   # 3/2 rule: padded length 45 is odd
 ])
 def test_plan_create_rejects(lib, kw, code, msg):
     h = C.c_void_p()
@@ -118,7 +119,7 @@ def test_perfmodel_matches_byte_table():
     (dict(nonlinear=0), "nonlinear = 0"),
     (dict(n=(256, 256, 256)), "static scratch"),
     (dict(n=(128, 128, 128), precision=1, dealias=1), "static scratch"),
-    (dict(n=(20, 16, 16)), "transform length"),
+    (dict(n=(14, 16, 16)), "transform length"),
 ])
 def test_noalloc_rejects_before_touching_the_device(lib, kw, msg):
     """The allocation-free entry point validates the descriptor and checks
