@@ -552,24 +552,29 @@ __device__ __forceinline__ void stage_terms(const StageArgs& a, long long idx, i
       rng_draw<T>(seed, stp, idx, e, w1, w2);
       return add<T>(scl<T>(w1, fac), scl<T>(scl<T>(w2, fac), na));
     };
+    bool projected = false;  // nz already orthogonal to k
     if (!raw && a.noise_type == 0) {
-      // On-device stream, thermal noise: draw the independent entries of the
-      // symmetric tensor directly.  W_ij + W_ji over sqrt 2 of two independent
-      // draws has the law of one draw, so (W + W^T) sqrt(1/2) is sampled with
-      // D(D+1)/2 draws per mode instead of D^2, with the same distribution.
-      int c = 0;
-#pragma unroll
-      for (int i = 0; i < D; ++i) {
-#pragma unroll
-        for (int j = i; j < D; ++j) {
-          const cplx<T> g = draw(c++);
-          // stored so that the symmetrisation below reproduces g off the
-          // diagonal and sqrt(2) g on it
-          const T h = (T)0.70710678118654752440;
-          w[i * D + j] = scl<T>(g, i == j ? (T)1 : h);
-          w[j * D + i] = w[i * D + j];
-        }
+      // On-device stream, thermal noise.  The reference projects the
+      // divergence of the symmetrised tensor, n = P (i W_sym k) (space_solver.py:
+      // 240-265 with math.py:72-100).  W_sym has E W_ij W_lm* = s2 (d_il d_jm +
+      // d_im d_jl) (entries of variance s2, 2 s2 on the diagonal), so i W_sym k
+      // has covariance s2 (k^2 I + k k^T) and n has s2 k^2 P, P = I - k k^T / k^2.
+      // i k x z, z a vector of independent entries of variance s2, has
+      // covariance s2 (k^2 I - k k^T) = s2 k^2 P as well and is orthogonal to k:
+      // the same Gaussian law from D = 3 draws per mode in 3D and one in 2D
+      // (z along the third axis) instead of D(D+1)/2, with no projection.
+      if constexpr (D == 3) {
+        const cplx<T> z0 = draw(0), z1 = draw(1), z2 = draw(2);
+        const T k0 = m.k[0], k1 = m.k[1], k2 = m.k[2];
+        nz[0] = mul_i<T>(sub<T>(scl<T>(z2, k1), scl<T>(z1, k2)));
+        nz[1] = mul_i<T>(sub<T>(scl<T>(z0, k2), scl<T>(z2, k0)));
+        nz[2] = mul_i<T>(sub<T>(scl<T>(z1, k0), scl<T>(z0, k1)));
+      } else {
+        const cplx<T> z = draw(0);
+        nz[0] = mul_i<T>(scl<T>(z, m.k[1]));
+        nz[1] = mul_i<T>(scl<T>(z, -m.k[0]));
       }
+      projected = true;
     } else if (!raw) {
 #pragma unroll
       for (int c = 0; c < D; ++c) w[c] = draw(c);
@@ -582,21 +587,24 @@ __device__ __forceinline__ void stage_terms(const StageArgs& a, long long idx, i
         }
       }
     }
-    if (a.noise_type == 0) {
-      const T h = (T)0.70710678118654752440;
+    if (!projected) {
+      if (a.noise_type == 0) {
+        // math.py / space_solver.py:15-64: i k . (W + W^T) sqrt(1/2)
+        const T h = (T)0.70710678118654752440;
 #pragma unroll
-      for (int i = 0; i < D; ++i) {
-        cplx<T> acc = mk<T>(0, 0);
+        for (int i = 0; i < D; ++i) {
+          cplx<T> acc = mk<T>(0, 0);
 #pragma unroll
-        for (int j = 0; j < D; ++j) acc = add<T>(acc, scl<T>(add<T>(w[i * D + j], w[j * D + i]), h * m.k[j]));
-        nz[i] = mul_i<T>(acc);
+          for (int j = 0; j < D; ++j) acc = add<T>(acc, scl<T>(add<T>(w[i * D + j], w[j * D + i]), h * m.k[j]));
+          nz[i] = mul_i<T>(acc);
+        }
+      } else {
+        const cplx<T> C = ((const cplx<T>*)a.corr)[idx];
+#pragma unroll
+        for (int i = 0; i < D; ++i) nz[i] = mul<T>(C, w[i]);
       }
-    } else {
-      const cplx<T> C = ((const cplx<T>*)a.corr)[idx];
-#pragma unroll
-      for (int i = 0; i < D; ++i) nz[i] = mul<T>(C, w[i]);
+      project<T, D>(nz, m);
     }
-    project<T, D>(nz, m);
     const T mag = (T)a.noise_mag;
 #pragma unroll
     for (int i = 0; i < D; ++i) nz[i] = scl<T>(nz[i], mag);

@@ -281,28 +281,35 @@ def test_anisotropic_box_rk3_thermal(mods, prec):
 
 
 # ------------------------------------ on-device noise stream vs the oracle
-def _single_draw_raw(seed, step, nk, shape, d, ntype):
+def _single_draw_raw(seed, step, nk, shape, d, ntype, k):
     """The oracle's raw draws (w1, w2) that reproduce one step of the device's
     single-draw stream (implicit PC Stokes, whose only live noise stage has
     a = 1): entry e of the stream is word pair e % 2 of counter group e // 2
-    (po.device_noise_draws), scaled by sqrt(1 + a^2) = sqrt 2.  Thermal:
-    entries are the upper triangle of the symmetric tensor in row order; the
-    oracle symmetrises w1 (space_solver.py:15-42), so w1 holds the tensor
-    with sqrt 2 on the diagonal and w2 = 0.  Correlated: w1 = sqrt 2 g."""
-    ne = d * (d + 1) // 2 if ntype == "thermal" else d
+    (po.device_noise_draws), scaled by sqrt(1 + a^2) = sqrt 2.
+
+    Thermal: the device draws z (3 entries in 3D, one in 2D, along the third
+    axis) and forms the projected noise as i k x z.  The oracle forms
+    P (i W_sym k) from w1 with W_sym = symmetrize(w1) = sqrt 2 w1 for a
+    symmetric w1 (space_solver.py:15-64), so w1 = (v k^T + k v^T) / (sqrt 2 k^2)
+    with v = sqrt 2 (k x z) gives W_sym k = v, which P leaves as it is
+    (v is orthogonal to k); w1 = 0 at k = 0.  Correlated: w1 = sqrt 2 g."""
+    ne = (3 if d == 3 else 1) if ntype == "thermal" else d
     groups = (ne + 1) // 2
     dev = po.device_noise_draws([seed], [step], nk, groups)[0]  # (2, groups, nk)
     g = [dev[e % 2, e // 2].reshape(shape) for e in range(ne)]
     if ntype == "thermal":
+        kk = [np.broadcast_to(k[i], shape) for i in range(d)]
+        if d == 3:
+            v = [kk[1] * g[2] - kk[2] * g[1], kk[2] * g[0] - kk[0] * g[2], kk[0] * g[1] - kk[1] * g[0]]
+        else:
+            v = [kk[1] * g[0], -kk[0] * g[0]]
+        v = [np.sqrt(2.0) * x for x in v]
+        k2 = sum(x * x for x in kk)
+        inv = np.where(k2 > 0, 1.0 / np.where(k2 > 0, k2, 1.0), 0.0) / np.sqrt(2.0)
         w1 = np.zeros((d * d,) + shape, dtype=complex)
-        e = 0
         for i in range(d):
-            for j in range(i, d):
-                if i == j:
-                    w1[i * d + i] = np.sqrt(2.0) * g[e]
-                else:
-                    w1[i * d + j] = w1[j * d + i] = g[e]
-                e += 1
+            for j in range(d):
+                w1[i * d + j] = (v[i] * kk[j] + kk[i] * v[j]) * inv
     else:
         w1 = np.sqrt(2.0) * np.array(g)
     return w1, np.zeros_like(w1)
@@ -331,7 +338,7 @@ def test_resident_stokes_device_noise_vs_oracle(mods, N, ntype):
         o.initialize(u0, space="fourier", project=True)
         starts.append(o.u_hat.copy())
         nk = int(np.prod(o.g.shape_fourier))
-        noise = [_single_draw_raw(seeds[b], s, nk, o.g.shape_fourier, d, ntype) for s in range(steps)]
+        noise = [_single_draw_raw(seeds[b], s, nk, o.g.shape_fourier, d, ntype, o.g.k) for s in range(steps)]
         o.solve(steps, noise=noise)
         refs.append(o.u_hat)
     if C is not None:
@@ -379,7 +386,7 @@ def test_served_requests_vs_oracle(mods):
         o.initialize(u0, space="fourier", project=False, mask_zero_mode=False)
         if sp.noise_mag:
             nk = int(np.prod(o.g.shape_fourier))
-            noise = [_single_draw_raw(r.seed, s, nk, o.g.shape_fourier, len(sp.n), sp.noise_type)
+            noise = [_single_draw_raw(r.seed, s, nk, o.g.shape_fourier, len(sp.n), sp.noise_type, o.g.k)
                      for s in range(r.steps)]
         else:
             noise = _silent(r.steps)

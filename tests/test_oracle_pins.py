@@ -291,3 +291,41 @@ def test_shell_spectrum_sums_to_physical_energy(N):
     E = po.shell_spectrum(g, uh)
     assert np.isclose(E.sum(), 0.5 * np.mean(np.sum(u**2, axis=0)), rtol=1e-13)
     assert np.isclose(po.energy(g, uh), 0.5 * np.mean(np.sum(u**2, axis=0)), rtol=1e-13)
+
+
+# ------------------------------------- the device's thermal-noise sampling
+@pytest.mark.parametrize("N", [(8, 6), (6, 4, 8)])
+def test_curl_form_thermal_noise_has_the_reference_law(N):
+    """The device samples thermal noise as i k x z (z: 3 independent complex
+    normals per mode in 3D, one along the third axis in 2D) instead of the
+    reference's P (i symmetrize(W) k) of D^2 of them.  Both are linear maps of
+    i.i.d. normals, so each is Gaussian with covariance A A^H for its map A.
+    Here A is built column by column from the oracle's own symmetrize,
+    noise_divergence and leray_projection applied to unit draws, and the two
+    covariances are compared exactly at every mode (equal to k^2 P)."""
+    g = po.Grid(N)
+    d = len(N)
+    k = g.k
+    shape = g.shape_fourier
+    cols = []
+    for c in range(d * d):
+        w = np.zeros((d * d,) + shape, dtype=complex)
+        w[c] = 1.0
+        n, _ = po.leray_projection(po.noise_divergence(po.symmetrize(w), k), k, g.k_over_k2)
+        cols.append(n)
+    A = np.stack(cols, axis=1)  # (d, d^2, *shape)
+    ref = np.einsum("ic...,jc...->ij...", A, A.conj())
+    if d == 3:
+        B = np.zeros((3, 3) + shape, dtype=complex)  # i k x e_c
+        for c in range(3):
+            e = np.zeros((3,) + shape)
+            e[c] = 1.0
+            B[:, c] = 1j * np.cross(k, e, axis=0)
+    else:
+        B = np.stack([1j * k[1], -1j * k[0]])[:, None]
+    cur = np.einsum("ic...,jc...->ij...", B, B.conj())
+    assert np.max(np.abs(cur - ref)) < 1e-12 * max(1.0, np.max(np.abs(ref)))
+    # and both equal k^2 P
+    k2 = g.k2
+    P = np.eye(d).reshape((d, d) + (1,) * d) * k2 - np.einsum("i...,j...->ij...", k, k)
+    assert np.max(np.abs(ref - P)) < 1e-12 * np.max(k2)
