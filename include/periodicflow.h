@@ -236,11 +236,17 @@ int pf_set_fusion(pf_plan* plan, int32_t fuse_stage);
  * are planned by shenfun/FFTW: fft_plan="FFTW_MEASURE", pf/simulation.py:76,
  * pf/basis.py:63-65).  tiles[PF_NPASS], indexed by PF_PASS_*: 0 is the
  * default tile width (column passes) or row-pair count (row passes); FP32
- * plans also have 1 (half) and 2 (twice) where they fit in LDS.  Plans of
- * Stokes flow with implicit PC and no forcing have PF_PASS_STAGE variant 1:
- * one launch per step instead of many steps with the state in registers.  A
- * choice changes only how work is mapped to threads: results are
- * bit-identical.
+ * plans also have 1 (half) and 2 (twice) where they fit in LDS, column
+ * variants 3 (two tiles per block), 4 (stage kinds: 1024-thread blocks of
+ * twice the width), 5 (the radix-2/3 plan), 6 (nontemporal stores, whole
+ * 128-byte lines stored by the axis-0 inverse and axis-1 forward passes) and
+ * 7 (nontemporal stores and loads), and row variants 3 (the radix-2/3 plan)
+ * and 4 (nontemporal stores).  Plans of Stokes flow with implicit PC and no
+ * forcing have PF_PASS_STAGE variant 1: one launch per step instead of many
+ * steps with the state in registers; other plans have PF_PASS_STAGE variant
+ * 7: the separate stage kernel with nontemporal loads and stores.  A choice
+ * other than a radix plan changes only how work is mapped to threads and
+ * cached: results are bit-identical.
  * pf_set_tiles fails with PF_EUNSUPPORTED for a variant that does not exist
  * at this plan's lengths; pf_get_tiles returns the plan's choices and
  * whether the stage pass is fused. */

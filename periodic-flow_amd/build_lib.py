@@ -23,11 +23,14 @@ SOURCES = [
     "pf_col_f32_v3.hip",
     "pf_col_f32_v4.hip",
     "pf_col_f32_v5.hip",
+    "pf_col_f32_v6.hip",
+    "pf_col_f32_v7.hip",
     "pf_col_f64.hip",
     "pf_row_f32_v0.hip",
     "pf_row_f32_v1.hip",
     "pf_row_f32_v2.hip",
     "pf_row_f32_v3.hip",
+    "pf_row_f32_v4.hip",
     "pf_row_f64.hip",
     "pf_api.hip",
     "pf_point.hip",

@@ -60,7 +60,7 @@ struct pf_plan {
   int M[3] = {1, 1, 1};
   int nh = 0;
   long long pitch = 0;           // row pitch of the work buffers (128-byte multiple)
-  long long dplane = 0;          // 3D: stride of D's axis-0 planes, N1 nh + one 128-byte line
+  long long dplane = 0;          // 3D: stride of D's axis-0 planes, N1 P + one 128-byte line
   long long aplane = 0;          // 3D: stride of A's axis-0 planes, N1 P + one 128-byte line
   long long nk = 0, nphys = 0;
   int B = 1;
@@ -291,9 +291,10 @@ int col_groups(pf_plan* p, int L, int kind, int v, ColArgs a, int ngroups, long 
                int nvalid, hipStream_t s, int cap_in = 0, int cap_out = 0, bool flat = false) {
   const int tc = tile_cols<T>(L, kind, v);
   if (tc <= 0) return -4;
-  // the fused stage kinds address the state by one flat column index
-  if ((kind == K_FWD_STAGE3 || kind == K_FWD_STAGE2) && ngroups != 1)
-    return fail(PF_EINVAL, "stage pass needs one flat column group");
+  // the fused stage kinds update every enumerated column: the groups must be
+  // enumerated back to back, exactly
+  if ((kind == K_FWD_STAGE3 || kind == K_FWD_STAGE2) && !flat)
+    return fail(PF_EINVAL, "stage pass needs flat column groups");
   ColGeom& g = a.g;
   g.n2c = flat ? nvalid : (nvalid + tc - 1) / tc * tc;
   g.cload = widen(nvalid, g.n2c, cap_in);
@@ -336,7 +337,7 @@ template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStrea
     const int N0 = p->N[0], N1 = p->N[1], N2 = p->N[2];
     const int M0 = p->M[0], M1 = p->M[1], M2 = p->M[2];
     const long long A_f = (long long)M0 * p->aplane, B_f = (long long)M0 * M1 * P;
-    const long long D_f = (long long)M0 * p->dplane;  // D keeps the state's row length
+    const long long D_f = (long long)M0 * p->dplane;  // D: rows of pitch P
     // retained last-axis entries (2/3 rule) or all nh of them (3/2 rule)
     const int lval = pr ? N2 / 3 + 1 : (int)nh;
     // P1: axis-0 inverse of u_hat and omega_hat, u_hat -> A (6 fields, w1)
@@ -363,10 +364,15 @@ template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStrea
       // The (j, l) columns are enumerated back to back, so no tile is
       // padded at a group's end (3/2 rule at 96^3: 49 entries per group,
       // which whole tiles per group rounded up to 64; 15% faster).
-      // one launch: blocks of fields 0..2 transform u_hat, blocks of 3..5 omega_hat
+      // one launch: blocks of fields 0..2 transform u_hat, blocks of 3..5 omega_hat.
+      // With nontemporal stores (whose partial lines cost a read-modify-write
+      // of the line) each group starts a new tile and whole tiles are stored
+      // into A's padded rows, so every store is whole 128-byte lines.
       a.f_base = 0;
       a.n_fields = 6;
-      PF_TRY(col_groups<T>(p, M0, K_INV_UC3, p->tsel[PF_PASS_INV0], a, ngroups, nh, P, lval, s, 0, 0, true));
+      const int v0 = p->tsel[PF_PASS_INV0];
+      const bool lines = col_cache_policy(v0) != 0;
+      PF_TRY(col_groups<T>(p, M0, K_INV_UC3, v0, a, ngroups, nh, P, lval, s, 0, lines ? (int)P : 0, !lines));
     }
     // P2: axis-1 inverse, A -> Bbuf (w2)
     a = col_args();
@@ -401,21 +407,22 @@ template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStrea
     r.n_members = B; r.n_fields = 1;
     r.mask23 = pr; r.scale = 1.0 / M2;
     { PassTimer t(p, PF_PASS_ROW, s); PF_TRY(row<T>(p, M2, ROW_FUSED3, p->tsel[PF_PASS_ROW], r, s)); }
-    // P4: axis-1 forward + truncation: C -> D (w2), all nh entries.  D has
-    // the state's layout, so the axis-0 forward reads and writes the same
-    // flat column index.
+    // P4: axis-1 forward + truncation: C -> D (w2), all nh entries in rows
+    // of pitch P.
     a = col_args();
     a.in = p->w1; a.out = p->w2;
     a.rows_in = M1; a.rows_out = N1;
-    a.in_row = P; a.out_row = nh;
+    a.in_row = P; a.out_row = P;
     a.in_field = B_f; a.out_field = D_f;
     a.in_member = 3 * B_f; a.out_member = 3 * D_f;
     a.n_fields = 3; a.n_members = B; a.scale = 1.0 / M1;
     {
       PassTimer t(p, PF_PASS_FWD1, s);
-      // whole tiles are loaded from C's padded rows; D's rows hold nh entries
-      PF_TRY(col_planes<T>(p, M1, K_FWD, p->tsel[PF_PASS_FWD1], a, M0, (long long)M1 * P, p->dplane, (int)nh,
-                           s, (int)P, 0));
+      // whole tiles are loaded from C's padded rows; D's rows get their nh
+      // entries, or whole tiles (whole lines) with nontemporal stores
+      const int v4 = p->tsel[PF_PASS_FWD1];
+      const int cap = col_cache_policy(v4) != 0 ? (int)P : 0;
+      PF_TRY(col_planes<T>(p, M1, K_FWD, v4, a, M0, (long long)M1 * P, p->dplane, (int)nh, s, (int)P, cap));
     }
   } else {
     const int N0 = p->N[0], N1 = p->N[1];
@@ -469,21 +476,22 @@ template <typename T> int last_forward(pf_plan* p, int kind, void* out, const St
   a.out_field = p->nk; a.out_member = d * p->nk;
   if (st) a.st = *st;
   if (d == 3) {
-    // D and the state share one row layout: one flat group of N1 * nh
-    // columns (D's planes are one cache line further apart, which keeps the
-    // tile's rows off a near-power-of-two stride)
+    // the N1 rows of nh entries of each plane enumerated back to back: D
+    // holds them at pitch P (its planes one cache line further apart, which
+    // keeps the tile's rows off a near-power-of-two stride), the state at nh
     const int N1 = p->N[1];
     const long long D_f = (long long)M0 * p->dplane;
     a.in_row = p->dplane; a.out_row = (long long)N1 * nh;
     a.in_field = D_f; a.in_member = d * D_f;
     a.gN1 = N1; a.gnh = (int)nh;
-    return col_groups<T>(p, M0, kind, v, a, 1, 0, 0, N1 * (int)nh, s);
+    return col_groups<T>(p, M0, kind, v, a, N1, P, nh, (int)nh, s, 0, 0, true);
   }
   const long long D_f = (long long)M0 * P;
   a.in_row = P; a.out_row = nh;
   a.in_field = D_f; a.in_member = d * D_f;
   a.gN1 = 1; a.gnh = (int)nh;
-  return col_groups<T>(p, M0, kind, v, a, 1, 0, 0, (int)nh, s);
+  // exactly the nh columns (the stage kinds update every enumerated column)
+  return col_groups<T>(p, M0, kind, v, a, 1, 0, 0, (int)nh, s, 0, 0, true);
 }
 
 // Nonlinear term (unprojected) of u_hat into the plan's R buffer (w1).
@@ -677,6 +685,10 @@ bool resident_steps(const pf_plan* p, const void* noise) {
   return corrector_only(p) && noise == nullptr && p->tsel[PF_PASS_STAGE] == 0;
 }
 
+// Cache policy of the separate stage kernel: nontemporal loads and stores
+// under PF_PASS_STAGE variant 7 (plans whose steps are not corrector-only).
+int stage_cache_policy(const pf_plan* p) { return p->tsel[PF_PASS_STAGE] == 7 ? 3 : 0; }
+
 constexpr int kResidentChunk = 200;  // steps per register-resident launch
 
 // n_steps corrector-only steps with the state in registers; acc (nullable)
@@ -726,7 +738,7 @@ template <typename T> int step_impl(pf_plan* p, void* uhat, long long n_steps, c
           if (rc) return rc;
         }
         PassTimer t(p, PF_PASS_STAGE, s);
-        PF_TRY(launch_stage<T>(p->dim, a, s));
+        PF_TRY(launch_stage<T>(p->dim, a, s, stage_cache_policy(p)));
       }
     }
     PF_TRY(launch_member_step(p->msteps, p->nact, s));
@@ -876,8 +888,10 @@ template <typename T> bool variant_ok(const pf_plan* p, int pass, int v) {
     case PF_PASS_FWD1: return d3 && tile_cols<T>(p->M[1], K_FWD, v) > 0;
     case PF_PASS_FWD0: return tile_cols<T>(p->M[0], K_FWD, v) > 0;
     case PF_PASS_FWD0_STAGE: return tile_cols<T>(p->M[0], d3 ? K_FWD_STAGE3 : K_FWD_STAGE2, v) > 0;
-    // corrector-only Stokes steps: 0 register-resident, 1 one launch per step
-    case PF_PASS_STAGE: return v == 1 && corrector_only(p);
+    // corrector-only Stokes steps: 0 register-resident, 1 one launch per
+    // step; other plans: 7 the separate stage kernel with nontemporal loads
+    // and stores
+    case PF_PASS_STAGE: return corrector_only(p) ? v == 1 : v == 7;
     default: return false;
   }
 }
@@ -904,7 +918,7 @@ struct DevBuffer {
 // variant), then the front passes again and the unfused axis-0 forward and
 // stage update, so that each candidate follows the same passes (whose output
 // may still sit in the last-level cache) as it does in a step.
-template <typename T> int tune_round(pf_plan* p, void* u, bool fused_ok, hipStream_t s) {
+template <typename T> int tune_round(pf_plan* p, void* u, bool fused_ok, int stage_cp, hipStream_t s) {
   const int ns = n_stages(p->d.integrator);
   const bool noise_on = p->d.noise_mag != 0.0 && uses_noise(p->d.integrator);
   StageArgs a = stage_args(p, u, ns > 1 ? 1 : 0, nullptr, noise_on);
@@ -919,7 +933,7 @@ template <typename T> int tune_round(pf_plan* p, void* u, bool fused_ok, hipStre
   }
   if ((rc = nonlinear_front<T>(p, u, s))) return rc;
   { PassTimer t(p, PF_PASS_FWD0, s); PF_TRY(last_forward<T>(p, K_FWD, p->w1, nullptr, s)); }
-  { PassTimer t(p, PF_PASS_STAGE, s); PF_TRY(launch_stage<T>(p->dim, a, s)); }
+  { PassTimer t(p, PF_PASS_STAGE, s); PF_TRY(launch_stage<T>(p->dim, a, s, stage_cp)); }
   return 0;
 }
 
@@ -939,14 +953,18 @@ template <typename T> int autotune_impl(pf_plan* p, const void* uhat, int reps, 
   double best[PF_NPASS], def[PF_NPASS];
   int bestv[PF_NPASS];
   for (int i = 0; i < PF_NPASS; ++i) { best[i] = def[i] = -1.0; bestv[i] = 0; }
-  double stage_ms = -1.0;
+  double stage_ms = -1.0, stage_def = -1.0;
+  int stage_cp = 0;
   int rc = 0;
   for (int v = -1; v < kVariants && !rc; ++v) {  // v = -1: warm-up round at the defaults
     const int vv = v < 0 ? 0 : v;
     for (int id : tunable) p->tsel[id] = variant_ok<T>(p, id, vv) ? vv : 0;
     const bool fused_ok = tile_cols<T>(p->M[0], p->dim == 3 ? K_FWD_STAGE3 : K_FWD_STAGE2, p->tsel[PF_PASS_FWD0_STAGE]) > 0;
+    // the separate stage kernel's cache policy follows the column variant's
+    // (its loads and stores nontemporal in the round of variant 7)
+    const int scp = col_cache_policy(vv) == 3 ? 3 : 0;
     pf_profile_enable(p, 1);
-    for (int r = 0; r < (v < 0 ? 1 : reps) && !rc; ++r) rc = tune_round<T>(p, tmp.ptr, fused_ok, s);
+    for (int r = 0; r < (v < 0 ? 1 : reps) && !rc; ++r) rc = tune_round<T>(p, tmp.ptr, fused_ok, scp, s);
     if (rc || v < 0) continue;
     for (int id : tunable) {
       if (!variant_ok<T>(p, id, v)) continue;
@@ -955,13 +973,15 @@ template <typename T> int autotune_impl(pf_plan* p, const void* uhat, int reps, 
       if (t >= 0.0 && (best[id] < 0.0 || t < best[id])) { best[id] = t; bestv[id] = v; }
     }
     const double ts = avg_ms(p, PF_PASS_STAGE);
-    if (ts >= 0.0 && (stage_ms < 0.0 || ts < stage_ms)) stage_ms = ts;
+    if (ts >= 0.0 && v == 0) stage_def = ts;
+    if (ts >= 0.0 && (stage_ms < 0.0 || ts < stage_ms)) { stage_ms = ts; stage_cp = scp; }
   }
   pf_profile_enable(p, prof0);
   p->prof_mask = mask0;
   const hipError_t e = hipStreamSynchronize(s);
   if (rc) {
     for (int id : tunable) p->tsel[id] = 0;
+    p->tsel[PF_PASS_STAGE] = 0;
     return rc;
   }
   if (e != hipSuccess) return fail(PF_EHIP, std::string("autotune: ") + hipGetErrorString(e));
@@ -975,6 +995,11 @@ template <typename T> int autotune_impl(pf_plan* p, const void* uhat, int reps, 
     }
     p->tsel[id] = bestv[id];
   }
+  if (stage_cp != 0 && stage_def >= 0.0 && stage_ms > kMargin * stage_def) {
+    stage_cp = 0;
+    stage_ms = stage_def;
+  }
+  p->tsel[PF_PASS_STAGE] = stage_cp ? 7 : 0;
   // fused against unfused axis-0 forward pass + stage update (fused unless
   // the unfused pair is clearly faster)
   if (best[PF_PASS_FWD0_STAGE] >= 0.0 && best[PF_PASS_FWD0] >= 0.0 && stage_ms >= 0.0)
@@ -1029,7 +1054,7 @@ int plan_geometry(pf_plan* p, const pf_desc& d, long long& w1c, long long& w2c) 
   const long long align = 128 / (long long)p->csz;
   p->pitch = (nh + align - 1) / align * align;
   const long long P = p->pitch;
-  p->dplane = d.dim == 3 ? (long long)p->N[1] * nh + align : 0;
+  p->dplane = d.dim == 3 ? (long long)p->N[1] * P + align : 0;
   p->aplane = d.dim == 3 ? (long long)p->N[1] * P + align : 0;
   w1c = B * d.dim * p->nk;
   w2c = 0;

@@ -86,11 +86,12 @@ template <typename T> __device__ __forceinline__ cplx<T> pack(cplx<T> x, cplx<T>
 
 // Row-pair variants of a row pass (the autotuner's choices): V = 0 the
 // default count, 1 half of it, 2 twice it, 3 the default count with radix
-// plan 1 (make_plan) where L has one; 0 where a variant repeats another or
-// does not fit in LDS.
+// plan 1 (make_plan) where L has one, 4 the default count with nontemporal
+// stores; 0 where a variant repeats another or does not fit in LDS.
 template <typename T, int MODE> constexpr int row_pairs_v(int L, int V) {
   const int np = row_pairs<T, MODE>(L);
   if (V == 3) return has_alt_plan(L) ? np : 0;
+  if (V == 4) return np;
   const int r = V == 0 ? np : (V == 1 ? np / 2 : np * 2);
   if (r < 1 || (V != 0 && r == np)) return 0;
   const long long lds = ((long long)r * row_slots<MODE>() * row_pitch(L) + L) * (long long)sizeof(cplx<T>);
@@ -105,7 +106,7 @@ template <typename T, int L, int MODE, int NP> constexpr int row_lds_buf() {
 
 // One block of a row pass: block (bx, by) of its launch; tws and buf are the
 // block's LDS (row_lds_tw and row_lds_buf entries).
-template <typename T, int L, int MODE, int NP, int RP = 0>
+template <typename T, int L, int MODE, int NP, int RP = 0, int CP = 0>
 __device__ __forceinline__ void row_body(const RowArgs& a, const cplx<T>* __restrict__ tw, cplx<T>* tws,
                                          cplx<T>* buf, int bx, int by) {
   constexpr int NS = row_slots<MODE>();
@@ -196,8 +197,8 @@ __device__ __forceinline__ void row_body(const RowArgs& a, const cplx<T>* __rest
         const int kk = i0 + (base + q) * TPS;
         X[q] = Y[q] = mk<T>(0, 0);
         if (!rows || base + q >= NIT || kk >= HK || kk > kmax) continue;
-        X[q] = px[kk];
-        Y[q] = py[kk];
+        X[q] = ld_stream<cp_load(CP), T>(px + kk);
+        Y[q] = ld_stream<cp_load(CP), T>(py + kk);
       }
 #pragma unroll
       for (int q = 0; q < CH; ++q) {
@@ -324,16 +325,16 @@ __device__ __forceinline__ void row_body(const RowArgs& a, const cplx<T>* __rest
     cplx<T> A = mk<T>((T)0.5 * (yk.x + ym.x), (T)0.5 * (yk.y - ym.y));
     cplx<T> B = mk<T>((T)0.5 * (yk.y + ym.y), (T)-0.5 * (yk.x - ym.x));
     if (L != N && k == N / 2) { A = mk<T>(2 * A.x, 0); B = mk<T>(2 * B.x, 0); }
-    oa[k] = scl<T>(A, s);
-    ob[k] = scl<T>(B, s);
+    st_stream<cp_store(CP), T>(oa + k, scl<T>(A, s));
+    st_stream<cp_store(CP), T>(ob + k, scl<T>(B, s));
   }
 }
 
-template <typename T, int L, int MODE, int NP, int RP = 0>
+template <typename T, int L, int MODE, int NP, int RP = 0, int CP = 0>
 __global__ void __launch_bounds__(kThreads) row_pass(RowArgs a, const cplx<T>* __restrict__ tw) {
   __shared__ cplx<T> tws[row_lds_tw<T, L, MODE, NP, RP>()];
   __shared__ cplx<T> buf[row_lds_buf<T, L, MODE, NP>()];
-  row_body<T, L, MODE, NP, RP>(a, tw, tws, buf, blockIdx.x, blockIdx.y);
+  row_body<T, L, MODE, NP, RP, CP>(a, tw, tws, buf, blockIdx.x, blockIdx.y);
 }
 
 // -------------------------------------------------------------- random numbers
@@ -669,7 +670,8 @@ __device__ __forceinline__ void stage_combine(const StageArgs& a, const cplx<T>*
 // holds it (fused forward pass), else it is read from a.rhs_nl.
 // upre / uzpre: u_hat and u0 of this mode already fetched (stage_fetch).
 // mpre: the mode's wavenumbers when the caller knows its grid indices.
-template <typename T, int D>
+// CP: cache policy of the u_hat / u0 stores.
+template <typename T, int D, int CP = 0>
 __device__ __forceinline__ void stage_point(const StageArgs& a, long long idx, int b, const cplx<T>* nlin,
                                             const cplx<T>* upre = nullptr, const cplx<T>* uzpre = nullptr,
                                             const Mode<T, D>* mpre = nullptr) {
@@ -686,7 +688,7 @@ __device__ __forceinline__ void stage_point(const StageArgs& a, long long idx, i
   }
   if (a.save_u0 == 1 && a.mode == 0) {
 #pragma unroll
-    for (int i = 0; i < D; ++i) u0p[i * a.nk] = u[i];
+    for (int i = 0; i < D; ++i) st_stream<cp_store(CP), T>(u0p + i * a.nk, u[i]);
   }
   const T Lk = a.lin_op ? ((const T*)a.lin_op)[idx] : (T)(-a.nu) * m.k2;
 
@@ -753,24 +755,26 @@ __device__ __forceinline__ void stage_point(const StageArgs& a, long long idx, i
   cplx<T> r[D];
   stage_combine<T, D>(a, u, uz, nl, nz, fr, Lk, r);
 #pragma unroll
-  for (int i = 0; i < D; ++i) uh[i * a.nk] = r[i];
+  for (int i = 0; i < D; ++i) st_stream<cp_store(CP), T>(uh + i * a.nk, r[i]);
 }
 
-// Loads of stage_point hoisted so that several modes' loads are in flight.
-template <typename T, int D>
+// Loads of stage_point hoisted so that several modes' loads are in flight
+// (CP: their cache policy).
+template <typename T, int D, int CP = 0>
 __device__ __forceinline__ void stage_fetch(const StageArgs& a, long long idx, int b, cplx<T>* u, cplx<T>* uz) {
   const long long mb = (long long)b * D * a.nk + idx;
 #pragma unroll
   for (int i = 0; i < D; ++i) {
-    u[i] = ((const cplx<T>*)a.uhat)[mb + i * a.nk];
-    uz[i] = (a.save_u0 || a.mode == 1) ? mk<T>(0, 0) : ((const cplx<T>*)a.u0)[mb + i * a.nk];
+    u[i] = ld_stream<cp_load(CP), T>((const cplx<T>*)a.uhat + mb + i * a.nk);
+    uz[i] = (a.save_u0 || a.mode == 1) ? mk<T>(0, 0) : ld_stream<cp_load(CP), T>((const cplx<T>*)a.u0 + mb + i * a.nk);
   }
 }
 
 // MPT modes per thread (block-strided, so each load instruction stays
 // coalesced): their u_hat / u0 loads are all issued first and their noise
-// chains interleave, which hides both memory and VALU latency.
-template <typename T, int D, int MPT>
+// chains interleave, which hides both memory and VALU latency.  CP: the
+// cache policy of the u_hat / u0 loads and stores.
+template <typename T, int D, int MPT, int CP = 0>
 __global__ void __launch_bounds__(kThreads) stage_kernel(StageArgs a) {
   const long long base = (long long)blockIdx.x * kThreads * MPT + threadIdx.x;
   const int b = blockIdx.y;
@@ -778,14 +782,14 @@ __global__ void __launch_bounds__(kThreads) stage_kernel(StageArgs a) {
 #pragma unroll
   for (int q = 0; q < MPT; ++q) {
     const long long idx = base + (long long)q * kThreads;
-    if (idx < a.nk && a.mode != 2) stage_fetch<T, D>(a, idx, b, uu[q], zz[q]);
+    if (idx < a.nk && a.mode != 2) stage_fetch<T, D, CP>(a, idx, b, uu[q], zz[q]);
   }
 #pragma unroll
   for (int q = 0; q < MPT; ++q) {
     const long long idx = base + (long long)q * kThreads;
     if (idx >= a.nk) continue;
     if (a.mode == 2) stage_point<T, D>(a, idx, b, nullptr);
-    else stage_point<T, D>(a, idx, b, nullptr, uu[q], zz[q]);
+    else stage_point<T, D, CP>(a, idx, b, nullptr, uu[q], zz[q]);
   }
 }
 
@@ -1076,11 +1080,12 @@ template <typename T, int KIND> constexpr int col_pipe_v(int L, int V) {
 // Launch variants (the autotuner's choices): V = 0 the default tile width,
 // 1 half of it, 2 twice it, 3 the default width with two tiles per block,
 // 4 (3D stage kind) twice the width with 1024-thread blocks, 5 the default
-// width with radix plan 1 (make_plan) where L has one; width 0 where a
-// variant does not fit in LDS, does not exist or would repeat variant 0.
+// width with radix plan 1 (make_plan) where L has one, 6 and 7 the default
+// width with nontemporal stores (7: and loads); width 0 where a variant does
+// not fit in LDS, does not exist or would repeat variant 0.
 template <typename T, int KIND> constexpr int col_width_v(int L, int V) {
   const int base = col_width<T, KIND>(L);
-  if (base == 0 || V == 0) return base;
+  if (base == 0 || V == 0 || V == 6 || V == 7) return base;
   if (V == 5) return has_alt_plan(L) ? base : 0;
   if (V == 3) return col_pipe_v<T, KIND>(L, 3) == 2 ? base : 0;
   // 1024-thread blocks cap a thread at 128 registers: the 2D stage update
@@ -1129,7 +1134,7 @@ template <typename T, int L, int TC, int KIND> constexpr int col_lds_buf() { ret
 
 // One block of a column pass: block bid of the nblk blocks of its launch;
 // tws and buf are the block's LDS (col_lds_tw and col_lds_buf entries).
-template <typename T, int L, int TC, int KIND, int PIPE, int NT, int RP = 0>
+template <typename T, int L, int TC, int KIND, int PIPE, int NT, int RP = 0, int CP = 0>
 __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __restrict__ tw, cplx<T>* tws,
                                          cplx<T>* buf, long long bid, long long nblk) {
   constexpr bool STAGE = KIND == K_FWD_STAGE3 || KIND == K_FWD_STAGE2;
@@ -1186,20 +1191,24 @@ __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __rest
   // LDS column cf -> field, tile column, (j, l), whether it is loaded (ld)
   // and stored (ok)
   struct Slot { int f, c, j, l; bool ld, ok; };
+  // the stage kinds' tile starts in group jb0 and spans at most
+  // ceil(TC / n2c) + 1 groups (PIPE = 1: c0 is fixed)
+  const int jb0 = STAGE ? c0 / gm.n2c : 0;
   auto slot = [&](int cf) {
     Slot r;
     r.f = STAGE ? cf / TC : fg;
     r.c = STAGE ? cf % TC : cf;
     const int cc = c0 + r.c;
+    const bool in = cc < gm.cols;
     if constexpr (STAGE) {
-      // the stage kinds run over one flat group (checked by the launcher)
-      r.j = 0;
-      r.l = cc;
-      r.ld = cc < gm.cload;
-      r.ok = cc < gm.cvalid;
+      // every column of every group is loaded and updated
+      int l = cc - jb0 * gm.n2c, j = jb0;
+      while (l >= gm.n2c) { l -= gm.n2c; ++j; }
+      r.j = j;
+      r.l = l;
+      r.ld = r.ok = in;
       return r;
     }
-    const bool in = cc < gm.cols;
     cpos(in ? cc : 0, r.j, r.l);
     r.ld = in && r.l < gm.cload;
     r.ok = in && r.l < gm.cvalid;
@@ -1253,16 +1262,16 @@ __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __rest
           continue;
         const cplx<T>* p = in + (long long)src * a.in_row + col;
         if constexpr (!CB) {
-          ra[q] = p[(long long)cfg * a.in_field];
+          ra[q] = ld_stream<cp_load(CP), T>(p + (long long)cfg * a.in_field);
         } else {
           int ia, ib;
           T ka, kb;
           curl_k(cfg, src, j, l, ia, ib, ka, kb);
-          ra[q] = p[(long long)ia * a.in_field];
-          rb[q] = p[(long long)ib * a.in_field];
+          ra[q] = ld_stream<cp_load(CP), T>(p + (long long)ia * a.in_field);
+          rb[q] = ld_stream<cp_load(CP), T>(p + (long long)ib * a.in_field);
         }
       } else {
-        ra[q] = in[(long long)f * a.in_field + (long long)src * a.in_row + col];
+        ra[q] = ld_stream<cp_load(CP), T>(in + (long long)f * a.in_field + (long long)src * a.in_row + col);
       }
     }
   };
@@ -1376,10 +1385,10 @@ __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __rest
               if (!cut) {
                 const cplx<T>* p = pcol + (long long)src * a.in_row;
                 if (!curlb) {
-                  ra = p[(long long)cfg * a.in_field];
+                  ra = ld_stream<cp_load(CP), T>(p + (long long)cfg * a.in_field);
                 } else {
-                  ra = p[(long long)ia * a.in_field];
-                  rb = p[(long long)ib * a.in_field];
+                  ra = ld_stream<cp_load(CP), T>(p + (long long)ia * a.in_field);
+                  rb = ld_stream<cp_load(CP), T>(p + (long long)ib * a.in_field);
                 }
               }
             }
@@ -1391,9 +1400,9 @@ __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __rest
               double fc;
               int src = pad_src(e, a.rows_in, L, fc);
               if (a.prune && src >= 0 && abs(cmode(src, a.rows_in)) > a.Krow) src = -1;
-              if (src >= 0) v = pc[(long long)src * a.in_row];
+              if (src >= 0) v = ld_stream<cp_load(CP), T>(pc + (long long)src * a.in_row);
             } else {
-              v = pc[(long long)e * a.in_row];
+              v = ld_stream<cp_load(CP), T>(pc + (long long)e * a.in_row);
             }
           }
           v0[kk][r] = v;
@@ -1474,7 +1483,7 @@ __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __rest
       if ((TOTL % NT == 0 || q < TOTL) && os.ok) {
         const int j = q / W;
 #pragma unroll
-        for (int r = 0; r < RL; ++r) out[(long long)(j + r * NSL) * a.out_row + ocol] = scl<T>(vl[kk][r], s);
+        for (int r = 0; r < RL; ++r) st_stream<cp_store(CP), T>(out + (long long)(j + r * NSL) * a.out_row + ocol, scl<T>(vl[kk][r], s));
       }
     }
   } else {
@@ -1492,9 +1501,10 @@ __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __rest
       // (2D: o * nh + cc).
       constexpr int SCH = 2;
       const long long rowlen = NFLD == 3 ? (long long)a.gN1 * a.gnh : (long long)a.gnh;
-      // TC divides the block size: one tile column per thread
+      // TC divides the block size: one tile column per thread.  The stage
+      // kinds enumerate the state's columns back to back (every one valid)
       const int c = threadIdx.x % TC;
-      const bool store = c0 + c < gm.cvalid;
+      const bool store = c0 + c < gm.cols;
       const long long mcol = c0 + c;
       // grid indices (j, l) of the column: one division per thread
       const int jt = NFLD == 3 ? (int)((unsigned)mcol / (unsigned)a.gnh) : 0;
@@ -1504,7 +1514,7 @@ __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __rest
 #pragma unroll
         for (int q = 0; q < SCH; ++q) {
           const int idx = idx0 + q * NT;
-          if (idx < n * TC && store) stage_fetch<T, NFLD>(a.st, (long long)(idx / TC) * rowlen + mcol, b, uu[q], zz[q]);
+          if (idx < n * TC && store) stage_fetch<T, NFLD, CP>(a.st, (long long)(idx / TC) * rowlen + mcol, b, uu[q], zz[q]);
         }
 #pragma unroll
         for (int q = 0; q < SCH; ++q) {
@@ -1515,7 +1525,7 @@ __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __rest
 #pragma unroll
           for (int f = 0; f < NFLD; ++f) nl[f] = trunc(o, f * TC + c);
           const Mode<T, NFLD> md = mode_at<T, NFLD>(a.st, o, jt, lt);
-          stage_point<T, NFLD>(a.st, (long long)o * rowlen + mcol, b, nl, uu[q], zz[q], &md);
+          stage_point<T, NFLD, CP>(a.st, (long long)o * rowlen + mcol, b, nl, uu[q], zz[q], &md);
         }
       }
     } else {
@@ -1526,7 +1536,7 @@ __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __rest
       for (int idx = threadIdx.x; idx < n * TC; idx += NT) {
         if (!os.ok) break;
         const int o = idx / TC;
-        out[(long long)o * a.out_row + ocol] = trunc(o, os.c);
+        st_stream<cp_store(CP), T>(out + (long long)o * a.out_row + ocol, trunc(o, os.c));
       }
     }
   }
@@ -1544,11 +1554,11 @@ __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __rest
 }
 
 
-template <typename T, int L, int TC, int KIND, int PIPE, int NT, int RP = 0>
+template <typename T, int L, int TC, int KIND, int PIPE, int NT, int RP = 0, int CP = 0>
 __global__ void __launch_bounds__(NT) col_pass(ColArgs a, const cplx<T>* __restrict__ tw) {
   __shared__ cplx<T> tws[col_lds_tw<T, L, TC, KIND, RP>()];
   __shared__ cplx<T> buf[col_lds_buf<T, L, TC, KIND>()];
-  col_body<T, L, TC, KIND, PIPE, NT, RP>(a, tw, tws, buf, blockIdx.x, gridDim.x);
+  col_body<T, L, TC, KIND, PIPE, NT, RP, CP>(a, tw, tws, buf, blockIdx.x, gridDim.x);
 }
 
 

@@ -17,13 +17,18 @@ bool length_supported(int L);
 // Launch variants: V = 0 is the default tile (column passes) or row-pair
 // count (row passes); FP32 also builds V = 1 (half), V = 2 (twice) and, for
 // column passes, V = 3 (two tiles per block, the second prefetched), V = 4
-// (stage kinds: 1024-thread blocks of twice the width) and V = 5 (radix
-// plan 1); for row passes V = 3 is radix plan 1.  These are the choices the
-// autotuner times.  One translation unit per precision and V.
-constexpr int kVariants = 6;
+// (stage kinds: 1024-thread blocks of twice the width), V = 5 (radix plan
+// 1), V = 6 (the default tile with nontemporal stores) and V = 7 (nontemporal
+// stores and loads); for row passes V = 3 is radix plan 1 and V = 4 the
+// default with nontemporal stores.  These are the choices the autotuner
+// times.  One translation unit per precision and V.
+constexpr int kVariants = 8;
 // the radix plan (make_plan) a launch variant uses
 constexpr int col_radix_plan(int v) { return v == 5 ? 1 : 0; }
 constexpr int row_radix_plan(int v) { return v == 3 ? 1 : 0; }
+// the cache policy (cp_store / cp_load) of a launch variant
+constexpr int col_cache_policy(int v) { return v == 6 ? 1 : (v == 7 ? 3 : 0); }
+constexpr int row_cache_policy(int v) { return v == 4 ? 1 : 0; }
 template <typename T, int V> int launch_col_v(int L, int kind, const ColArgs& a, const void* tw, hipStream_t s);
 template <typename T, int V> int col_width_q(int L, int kind);
 template <typename T, int V> int launch_row_v(int L, int mode, const RowArgs& a, const void* tw, hipStream_t s);
@@ -42,6 +47,8 @@ int launch_col(int L, int kind, const ColArgs& a, const void* tw, hipStream_t s,
       case 3: return launch_col_v<T, 3>(L, kind, a, tw, s);
       case 4: return launch_col_v<T, 4>(L, kind, a, tw, s);
       case 5: return launch_col_v<T, 5>(L, kind, a, tw, s);
+      case 6: return launch_col_v<T, 6>(L, kind, a, tw, s);
+      case 7: return launch_col_v<T, 7>(L, kind, a, tw, s);
       default: return -1;
     }
   } else {
@@ -58,6 +65,8 @@ template <typename T> int tile_cols(int L, int kind, int v) {
       case 3: return col_width_q<T, 3>(L, kind);
       case 4: return col_width_q<T, 4>(L, kind);
       case 5: return col_width_q<T, 5>(L, kind);
+      case 6: return col_width_q<T, 6>(L, kind);
+      case 7: return col_width_q<T, 7>(L, kind);
       default: return 0;
     }
   } else {
@@ -73,8 +82,10 @@ int launch_row(int L, int mode, const RowArgs& a, const void* tw, hipStream_t s,
       case 1: return launch_row_v<T, 1>(L, mode, a, tw, s);
       case 2: return launch_row_v<T, 2>(L, mode, a, tw, s);
       case 3: return launch_row_v<T, 3>(L, mode, a, tw, s);
-      case 4:
-      case 5: return -4;  // row passes have no such variants
+      case 4: return launch_row_v<T, 4>(L, mode, a, tw, s);
+      case 5:
+      case 6:
+      case 7: return -4;  // row passes have no such variants
       default: return -1;
     }
   } else {
@@ -89,6 +100,7 @@ template <typename T> int row_pair_count(int L, int mode, int v) {
       case 1: return row_pairs_q<T, 1>(L, mode);
       case 2: return row_pairs_q<T, 2>(L, mode);
       case 3: return row_pairs_q<T, 3>(L, mode);
+      case 4: return row_pairs_q<T, 4>(L, mode);
       default: return 0;
     }
   } else {
@@ -106,7 +118,7 @@ template <typename T>
 int launch_row_pair(int mode, int L0, const RowArgs& a0, const void* tw0, int L1, const RowArgs& a1, const void* tw1,
                     hipStream_t s);
 
-template <typename T> int launch_stage(int dim, const StageArgs& a, hipStream_t s);
+template <typename T> int launch_stage(int dim, const StageArgs& a, hipStream_t s, int cp = 0);
 template <typename T> int launch_diag(int dim, const DiagArgs& a, int n_members, hipStream_t s);
 template <typename T> int launch_spectrum(int dim, const DiagArgs& a, int n_members, int nshell, hipStream_t s);
 template <typename T> int launch_band(int dim, const BandArgs& a, int n_members, hipStream_t s);

@@ -42,15 +42,21 @@ static dim3 mode_grid(long long nk, int nmem) {
 // Modes per thread: four in 2D when the launch has enough modes to fill
 // the GPU several times over (the noise chains of Stokes flow then
 // interleave), else one; one in 3D (where four would halve the occupancy).
-template <typename T> int launch_stage(int dim, const StageArgs& a, hipStream_t s) {
+// cp: cache policy of the state loads and stores (0, or 3: nontemporal).
+template <typename T, int CP> static void stage_launch(int dim, const StageArgs& a, hipStream_t s) {
   auto grid = [&](int mpt) {
     return dim3((unsigned)((a.nk + (long long)kThreads * mpt - 1) / ((long long)kThreads * mpt)),
                 (unsigned)a.n_members);
   };
   const bool wide = a.nk * a.n_members >= (4ll << 20);
-  if (dim == 3) hipLaunchKernelGGL((stage_kernel<T, 3, 1>), grid(1), dim3(kThreads), 0, s, a);
-  else if (wide) hipLaunchKernelGGL((stage_kernel<T, 2, 4>), grid(4), dim3(kThreads), 0, s, a);
-  else hipLaunchKernelGGL((stage_kernel<T, 2, 1>), grid(1), dim3(kThreads), 0, s, a);
+  if (dim == 3) hipLaunchKernelGGL((stage_kernel<T, 3, 1, CP>), grid(1), dim3(kThreads), 0, s, a);
+  else if (wide) hipLaunchKernelGGL((stage_kernel<T, 2, 4, CP>), grid(4), dim3(kThreads), 0, s, a);
+  else hipLaunchKernelGGL((stage_kernel<T, 2, 1, CP>), grid(1), dim3(kThreads), 0, s, a);
+}
+template <typename T> int launch_stage(int dim, const StageArgs& a, hipStream_t s, int cp) {
+  if (cp == 3) stage_launch<T, 3>(dim, a, s);
+  else if (cp == 0) stage_launch<T, 0>(dim, a, s);
+  else return -1;
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }
 
@@ -144,7 +150,7 @@ template <typename T> int launch_stokes_steps(int dim, const StageArgs& a, int n
 }
 
 #define PF_INST(T)                                                                                    \
-  template int launch_stage<T>(int, const StageArgs&, hipStream_t);                                   \
+  template int launch_stage<T>(int, const StageArgs&, hipStream_t, int);                              \
   template int launch_project<T>(int, const StageArgs&, const void*, void*, int, int, int, hipStream_t); \
   template int launch_diag<T>(int, const DiagArgs&, int, hipStream_t);                                \
   template int launch_spectrum<T>(int, const DiagArgs&, int, int, hipStream_t);                       \

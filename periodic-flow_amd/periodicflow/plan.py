@@ -264,10 +264,15 @@ class Plan:
 
     def set_tiles(self, tiles):
         """Launch variant per pass (dict by pass name, unnamed passes 0): 0 the
-        default tile width or row-pair count, 1 half of it, 2 twice it; for
-        Stokes flow with implicit PC, "stage": 1 steps one launch per step
-        instead of many steps with the state in registers.  Results are
-        bit-identical across variants."""
+        default tile width or row-pair count, 1 half of it, 2 twice it; column
+        passes also 3 (two tiles per block), 4 (stage kinds: 1024-thread
+        blocks), 5 (the radix-2/3 plan), 6 (nontemporal stores) and 7
+        (nontemporal stores and loads); row passes also 3 (the radix-2/3
+        plan) and 4 (nontemporal stores).  For Stokes flow with implicit PC,
+        "stage": 1 steps one launch per step instead of many steps with the
+        state in registers; for other plans "stage": 7 gives the separate
+        stage kernel nontemporal loads and stores.  Results are bit-identical
+        across variants except the radix plans, which change rounding only."""
         n = len(_lib.PASS_NAMES)
         unknown = set(tiles) - set(_lib.PASS_NAMES)
         if unknown:

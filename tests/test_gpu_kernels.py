@@ -272,9 +272,11 @@ def _rel(a, b):
                                              ((16, 24, 32), "3/2", "rk2"), ((64, 64), "3/2", "rk3"),
                                              ((48, 64), "2/3", "rk4")])
 def test_launch_variants_bit_identical(pf, N, dealias, integ):
-    """Every geometry variant of every pass (tile width / rows per block)
+    """Every geometry or cache-policy variant of every pass (tile width,
+    rows per block, nontemporal loads / stores with whole-line geometry)
     maps the same arithmetic to other threads: the right-hand side and the
-    fused step agree bit for bit with the default plan.  Radix-plan variants
+    fused step agree bit for bit with the default plan, and so does the
+    unfused step with the nontemporal stage kernel.  Radix-plan variants
     (another factorisation of the transform length) agree to FP32 rounding.
     Autotune leaves u_hat as it was and picks only existing variants."""
     _say("variants", N, dealias, integ)
@@ -291,7 +293,7 @@ def test_launch_variants_bit_identical(pf, N, dealias, integ):
     tried = 0
     radix_tried = 0
     for name in TUNABLE:
-        for v in (1, 2, 3, 4, 5):
+        for v in (1, 2, 3, 4, 5, 6, 7):
             try:
                 plan.set_tiles({name: v})
             except NotImplementedError:
@@ -309,6 +311,18 @@ def test_launch_variants_bit_identical(pf, N, dealias, integ):
                 assert torch.equal(rhs, ref_rhs), (name, v)
                 assert torch.equal(w, ref_step), (name, v)
     assert tried > 0
+    # the separate stage kernel with nontemporal loads and stores
+    plan.set_fusion(False)
+    plan.set_tiles({})
+    ref_unfused = u.clone()
+    plan.step_counter = 0
+    plan.step(ref_unfused, 2)
+    plan.set_tiles({"stage": 7, "fwd_axis0": 7})
+    w = u.clone()
+    plan.step_counter = 0
+    plan.step(w, 2)
+    assert torch.equal(w, ref_unfused)
+    plan.set_fusion(True)
     if dealias == "3/2" and N[0] in (24, 16):
         assert radix_tried > 0  # padded lengths 36 / 24 ... have a second radix plan
     plan.set_tiles({})

@@ -125,7 +125,7 @@ def test_launch_variants_bit_identical(pf, N, dealias):
     plan.rhs_nonlinear(u, ref)
     tried = 0
     for name in _lib.PASS_NAMES:
-        for v in (1, 2, 3, 4, 5):
+        for v in (1, 2, 3, 4, 5, 6, 7):
             try:
                 plan.set_tiles({name: v})
             except (NotImplementedError, ValueError):
