@@ -346,6 +346,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch plans")
+    ap.add_argument("--plan", default=None,
+                    help="launch plan instead of autotuning, as printed in launch_plan: "
+                         "'pass=variant,...' and optionally 'fused=0|1' (e.g. to profile a tuned plan)")
     ap.add_argument("--suite", default=None, help="also run the long configurations, results to this JSON file")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 1 GiB device-to-device copy (the counter calibration kernel)")
@@ -366,7 +369,13 @@ def main():
     uh = u_init.clone()
     tune_s = None
     tune_ms = {}
-    if not args.no_autotune:
+    if args.plan:
+        kv = dict(item.split("=") for item in args.plan.split(",") if item)
+        fused_arg = kv.pop("fused", None)
+        plan.set_tiles({k: int(v) for k, v in kv.items()})
+        if fused_arg is not None:
+            plan.set_fusion(bool(int(fused_arg)))
+    elif not args.no_autotune:
         # launch plans chosen by timing (a one-off, like FFTW_MEASURE planning)
         torch.cuda.synchronize()
         ta = time.perf_counter()
@@ -427,7 +436,9 @@ def main():
         with open(tf) as f:
             t = json.load(f)
         traffic = t.get(dom)
-    step_b = perfmodel.step_bytes([N] * 3, "2/3", "fp32", 4, fused=fused)
+    # the step's roofline: the fewest algorithmic bytes a step moves (the
+    # fused sequence), whichever sequence the launch plan runs
+    step_b = perfmodel.step_bytes([N] * 3, "2/3", "fp32", 4, fused=True)
 
     if args.calibrate_copy:
         src = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
@@ -483,6 +494,7 @@ def main():
             "algorithmic_bytes": pb[dom],
         },
         "step_roofline_frac": round(step_b / (sec / args.steps) / 1e9 / perfmodel.HBM_PEAK_GBS, 4),
+        "step_bytes": step_b,
         "passes": per_pass,
         "launch_plan": {"tiles": {k: v for k, v in tiles.items() if v}, "fused_stage": fused_plan,
                         "autotune_s": None if tune_s is None else round(tune_s, 3),

@@ -752,8 +752,9 @@ constexpr int kMaxMulti = 16;  // plans one pf_step_multi call advances together
 // Issues the recorded launches of several plans pass by pass (record r of
 // every plan before record r + 1 of any, all on one stream, so each plan's
 // passes keep their order): two plans whose records at r are the same pass
-// kind at different compiled lengths, both at the default launch variant,
-// share one paired launch; the rest launch one by one.
+// kind at different compiled lengths, both at the same launch variant of the
+// default geometry (pair_variant: any cache policy), share one paired
+// launch; the rest launch one by one.
 template <typename T> int issue_logs(LaunchLog* logs, int np, hipStream_t s, int* paired) {
   const int nr = logs[0].n;
   for (int r = 0; r < nr; ++r) {
@@ -762,18 +763,18 @@ template <typename T> int issue_logs(LaunchLog* logs, int np, hipStream_t s, int
       if (used[i]) continue;
       const LaunchRec& a = logs[i].r[r];
       int partner = -1;
-      if (sizeof(T) == 4 && a.v == 0) {
+      if (sizeof(T) == 4 && pair_variant(a.row, a.v)) {
         for (int j = i + 1; j < np && partner < 0; ++j) {
           const LaunchRec& b = logs[j].r[r];
-          if (!used[j] && b.v == 0 && b.row == a.row && b.kind == a.kind && b.L != a.L && pair_compiled(a.L, b.L))
+          if (!used[j] && b.v == a.v && b.row == a.row && b.kind == a.kind && b.L != a.L && pair_compiled(a.L, b.L))
             partner = j;
         }
       }
       int rc = -4;
       if (partner >= 0) {
         const LaunchRec& b = logs[partner].r[r];
-        rc = a.row ? launch_row_pair<T>(a.kind, a.L, a.ra, a.tw, b.L, b.ra, b.tw, s)
-                   : launch_col_pair<T>(a.kind, a.L, a.ca, a.tw, b.L, b.ca, b.tw, s);
+        rc = a.row ? launch_row_pair<T>(a.kind, a.L, a.ra, a.tw, b.L, b.ra, b.tw, s, a.v)
+                   : launch_col_pair<T>(a.kind, a.L, a.ca, a.tw, b.L, b.ca, b.tw, s, a.v);
         if (rc == 0) {
           used[partner] = true;
           if (paired) ++*paired;

@@ -1575,7 +1575,7 @@ struct ColSeg {
   long long nblk;
 };
 
-template <typename T, int L0, int TC0, int L1, int TC1, int KIND>
+template <typename T, int L0, int TC0, int L1, int TC1, int KIND, int CP = 0>
 __global__ void __launch_bounds__(kThreads) col_pass_pair(ColSeg s0, ColSeg s1, long long split) {
   constexpr int TW0 = col_lds_tw<T, L0, TC0, KIND>(), TW1 = col_lds_tw<T, L1, TC1, KIND>();
   constexpr int BF0 = col_lds_buf<T, L0, TC0, KIND>(), BF1 = col_lds_buf<T, L1, TC1, KIND>();
@@ -1583,9 +1583,10 @@ __global__ void __launch_bounds__(kThreads) col_pass_pair(ColSeg s0, ColSeg s1, 
   __shared__ cplx<T> buf[BF0 > BF1 ? BF0 : BF1];
   const long long bid = blockIdx.x;
   if (bid < split) {
-    if (bid < s0.nblk) col_body<T, L0, TC0, KIND, 1, kThreads>(s0.a, (const cplx<T>*)s0.tw, tws, buf, bid, s0.nblk);
+    if (bid < s0.nblk)
+      col_body<T, L0, TC0, KIND, 1, kThreads, 0, CP>(s0.a, (const cplx<T>*)s0.tw, tws, buf, bid, s0.nblk);
   } else {
-    col_body<T, L1, TC1, KIND, 1, kThreads>(s1.a, (const cplx<T>*)s1.tw, tws, buf, bid - split, s1.nblk);
+    col_body<T, L1, TC1, KIND, 1, kThreads, 0, CP>(s1.a, (const cplx<T>*)s1.tw, tws, buf, bid - split, s1.nblk);
   }
 }
 
@@ -1595,7 +1596,7 @@ struct RowSeg {
   int gx, gy;
 };
 
-template <typename T, int L0, int NP0, int L1, int NP1, int MODE>
+template <typename T, int L0, int NP0, int L1, int NP1, int MODE, int CP = 0>
 __global__ void __launch_bounds__(kThreads) row_pass_pair(RowSeg s0, RowSeg s1, long long split) {
   constexpr int TW0 = row_lds_tw<T, L0, MODE, NP0>(), TW1 = row_lds_tw<T, L1, MODE, NP1>();
   constexpr int BF0 = row_lds_buf<T, L0, MODE, NP0>(), BF1 = row_lds_buf<T, L1, MODE, NP1>();
@@ -1604,10 +1605,11 @@ __global__ void __launch_bounds__(kThreads) row_pass_pair(RowSeg s0, RowSeg s1, 
   const long long bid = blockIdx.x;
   if (bid < split) {
     if (bid < (long long)s0.gx * s0.gy)
-      row_body<T, L0, MODE, NP0>(s0.a, (const cplx<T>*)s0.tw, tws, buf, (int)(bid % s0.gx), (int)(bid / s0.gx));
+      row_body<T, L0, MODE, NP0, 0, CP>(s0.a, (const cplx<T>*)s0.tw, tws, buf, (int)(bid % s0.gx),
+                                        (int)(bid / s0.gx));
   } else {
     const long long b1 = bid - split;
-    row_body<T, L1, MODE, NP1>(s1.a, (const cplx<T>*)s1.tw, tws, buf, (int)(b1 % s1.gx), (int)(b1 / s1.gx));
+    row_body<T, L1, MODE, NP1, 0, CP>(s1.a, (const cplx<T>*)s1.tw, tws, buf, (int)(b1 % s1.gx), (int)(b1 / s1.gx));
   }
 }
 

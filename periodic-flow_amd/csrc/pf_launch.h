@@ -108,15 +108,17 @@ template <typename T> int row_pair_count(int L, int mode, int v) {
   }
 }
 
-// Paired launches of two plans' passes (pf_pair.hip): FP32, default launch
-// variant, compiled length pairs only; -4 where a pair is not compiled.
+// Paired launches of two plans' passes (pf_pair.hip): FP32, both records at
+// launch variant v, which pair_variant admits (the default geometry with any
+// cache policy), compiled length pairs only; -4 where a pair is not compiled.
 bool pair_compiled(int L0, int L1);
+bool pair_variant(bool row, int v);
 template <typename T>
 int launch_col_pair(int kind, int L0, const ColArgs& a0, const void* tw0, int L1, const ColArgs& a1, const void* tw1,
-                    hipStream_t s);
+                    hipStream_t s, int v);
 template <typename T>
 int launch_row_pair(int mode, int L0, const RowArgs& a0, const void* tw0, int L1, const RowArgs& a1, const void* tw1,
-                    hipStream_t s);
+                    hipStream_t s, int v);
 
 template <typename T> int launch_stage(int dim, const StageArgs& a, hipStream_t s, int cp = 0);
 template <typename T> int launch_diag(int dim, const DiagArgs& a, int n_members, hipStream_t s);

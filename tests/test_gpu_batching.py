@@ -227,3 +227,31 @@ def test_paired_launches_equal_separate_plans(mods, prec):
         p.step(st, 3)
         assert torch.equal(st[0], ens.state(i)), i
         assert ens.plan_of(i).step_counter == 3
+
+
+def test_paired_launches_cache_policy(mods):
+    """Paired launches at the nontemporal launch variants (both groups at the
+    same variant of every pass): still one paired launch per pass, and the
+    members equal each plan stepped alone at the default variants."""
+    ensemble, plan, _ = mods
+    _say("paired launches, nontemporal variants")
+    S = ensemble.MemberSpec
+    members = [(S(n=(128,) * 3, dt=5e-3, integrator="rk3", viscosity=0.005, noise_mag=0.01), 31),
+               (S(n=(96,) * 3, dt=5e-3, integrator="rk3", viscosity=0.005, noise_mag=0.01), 32)]
+    ens = ensemble.Ensemble(members, precision="fp32")
+    inits = []
+    for i, (spec, seed) in enumerate(members):
+        u = ensemble.random_solenoidal(ens.plan_of(i), 50 + i, energy=0.5)
+        inits.append(u)
+        ens.set_fourier(i, u)
+    nt = {"inv_axis0_curl": 6, "inv_axis1": 7, "row_c2r_cross_r2c": 4, "fwd_axis1": 6, "fwd_axis0_stage": 6}
+    for g in ens.groups.values():
+        g.plan.set_tiles(nt)
+    paired = ens.step(2)
+    assert paired == 30, paired  # 2 steps x 3 stages x 5 passes
+    for i, (spec, seed) in enumerate(members):
+        p = plan.Plan(spec.n, spec.dt, precision="fp32", integrator="rk3", viscosity=spec.viscosity,
+                      noise_mag=spec.noise_mag, seeds=[seed])
+        st = torch.as_tensor(inits[i][None], device="cuda", dtype=p.complex_dtype).contiguous()
+        p.step(st, 2)
+        assert torch.equal(st[0], ens.state(i)), i
