@@ -430,12 +430,18 @@ def main():
     # mean over the launches the dominant pass recorded in the timed region
     dom_avg_s = prof[dom][0] / max(1, prof[dom][1]) * 1e-3
     achieved = pb[dom] / dom_avg_s / 1e9
-    traffic = None
+    # HBM bytes per launch of the dominant pass, from the committed profile
+    # of this workload (rocprofv3 counters need a run of their own); None
+    # when that profile's plan ran another variant of the pass
+    traffic, traffic_plan = None, None
     tf = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tf):
         with open(tf) as f:
             t = json.load(f)
-        traffic = t.get(dom)
+        traffic_plan = t.get("plan", "")
+        prof_v = dict(kv.split("=") for kv in traffic_plan.split(",") if kv)
+        if int(prof_v.get(dom, 0)) == tiles.get(dom, 0):
+            traffic = t.get("bytes_per_launch", {}).get(dom)
     # the step's roofline: the fewest algorithmic bytes a step moves (the
     # fused sequence), whichever sequence the launch plan runs
     step_b = perfmodel.step_bytes([N] * 3, "2/3", "fp32", 4, fused=True)
@@ -489,8 +495,9 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / perfmodel.HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "traffic_source": "profiles/traffic.json: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this workload, "
-                              "scaled by a 1 GiB copy's counters (an earlier profiling run, not this one)",
+            "traffic_source": "profiles/traffic.json: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this workload "
+                              f"with the launch plan {traffic_plan}, scaled by a 1 GiB copy's counters (an earlier "
+                              "profiling run, not this one)",
             "algorithmic_bytes": pb[dom],
         },
         "step_roofline_frac": round(step_b / (sec / args.steps) / 1e9 / perfmodel.HBM_PEAK_GBS, 4),
