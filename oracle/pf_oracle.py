@@ -403,6 +403,26 @@ def device_noise_draws(seeds, steps, nk, nc):
     return out
 
 
+def device_single_draws(seed, step, nk, ne):
+    """Normals (ne, nk) of one step of the on-device single-draw stream (a step
+    whose draws feed one stage): entry e of step s is word pair s % 2 (words
+    0-1 at even s, 2-3 at odd s, through Box-Muller) of the call at counter
+    (idx lo, idx hi ^ e << 24 ^ 2^31, (s // 2) lo, (s // 2) hi), key (seed lo,
+    seed hi): one call serves two consecutive steps."""
+    idx = np.arange(nk, dtype=np.uint64)
+    seed, step = int(seed), int(step)
+    n = step >> 1
+    out = np.zeros((ne, nk), dtype=complex)
+    for e in range(ne):
+        ctr = (idx & _M32, (idx >> np.uint64(32)) ^ np.uint64((e << 24) ^ 0x80000000),
+               np.full(nk, n & 0xFFFFFFFF, dtype=np.uint64), np.full(nk, (n >> 32) & 0xFFFFFFFF, dtype=np.uint64))
+        r = philox4x32_10(ctr, (seed & 0xFFFFFFFF, seed >> 32))
+        a, b = (r[0], r[1]) if step % 2 == 0 else (r[2], r[3])
+        g0, g1 = _box_muller(a, b)
+        out[e] = g0 + 1j * g1
+    return out
+
+
 class NoiseStream:
     """Reproducible replacement for the reference's global MT19937 stream.
 

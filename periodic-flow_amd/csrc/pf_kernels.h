@@ -377,18 +377,37 @@ __device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, double& g0, d
   g1 = r * sn;
 }
 
-// raw draws w1[c], w2[c] of mode idx in a step: four normals per call
+// two complex normals (words 0-1 and 2-3 through Box-Muller) of one
+// Philox4x32-10 call at counter (idx lo, idx hi ^ tag, n lo, n hi), key seed
 template <typename T>
-__device__ __forceinline__ void rng_draw(unsigned long long seed, long long step, long long idx, int c,
-                                         cplx<T>& w1, cplx<T>& w2) {
-  U4 ctr{(uint32_t)idx, (uint32_t)((unsigned long long)idx >> 32) ^ ((uint32_t)c << 24), (uint32_t)step,
-         (uint32_t)((unsigned long long)step >> 32)};
+__device__ __forceinline__ void rng_normals(unsigned long long seed, long long n, long long idx, uint32_t tag,
+                                            cplx<T>& w1, cplx<T>& w2) {
+  U4 ctr{(uint32_t)idx, (uint32_t)((unsigned long long)idx >> 32) ^ tag, (uint32_t)n,
+         (uint32_t)((unsigned long long)n >> 32)};
   const U4 r = philox(ctr, (uint32_t)seed, (uint32_t)(seed >> 32));
   T g0, g1, g2, g3;
   box_muller(r.x, r.y, g0, g1);
   box_muller(r.z, r.w, g2, g3);
   w1 = mk<T>(g0, g1);
   w2 = mk<T>(g2, g3);
+}
+
+// raw draws w1[c], w2[c] of mode idx in a step: counter group c at the step
+template <typename T>
+__device__ __forceinline__ void rng_draw(unsigned long long seed, long long step, long long idx, int c,
+                                         cplx<T>& w1, cplx<T>& w2) {
+  rng_normals<T>(seed, step, idx, (uint32_t)c << 24, w1, w2);
+}
+
+// The single-draw stream (a step whose draws feed one stage): entry e of
+// step stp is word pair stp & 1 of counter group e at the step pair stp >> 1,
+// so one call serves two consecutive steps (even: the first pair, odd: the
+// second).  The top bit of the counter's second word keeps these counters
+// apart from rng_draw's.
+template <typename T>
+__device__ __forceinline__ void rng_pair(unsigned long long seed, long long stp, long long idx, int e,
+                                         cplx<T>& even, cplx<T>& odd) {
+  rng_normals<T>(seed, stp >> 1, idx, ((uint32_t)e << 24) ^ 0x80000000u, even, odd);
 }
 
 // raw draws (B, 2, nc, nk) of one step: the device counterpart of random_fields
@@ -504,11 +523,13 @@ template <typename T, int D> __device__ __forceinline__ void project(cplx<T>* v,
 // step stp (space_solver.py:240-300): nl projected (math.py:72-100) and
 // Nyquist-masked, nz the projected noise scaled by noise_mag, fr the band
 // forcing.  nlin: the unprojected nonlinear term when the caller holds it,
-// else it is read from a.rhs_nl (or zero without a nonlinear term).
+// else it is read from a.rhs_nl (or zero without a nonlinear term).  zpre:
+// this step's single-draw normals when the caller drew them (rng_pair), else
+// they are drawn here.
 template <typename T, int D>
 __device__ __forceinline__ void stage_terms(const StageArgs& a, long long idx, int b, const cplx<T>* nlin,
                                             const cplx<T>* u, const Mode<T, D>& m, long long stp, cplx<T>* nl,
-                                            cplx<T>* nz, cplx<T>* fr) {
+                                            cplx<T>* nz, cplx<T>* fr, const cplx<T>* zpre = nullptr) {
   const long long mb = (long long)b * D * a.nk;
 #pragma unroll
   for (int i = 0; i < D; ++i) {
@@ -542,13 +563,14 @@ __device__ __forceinline__ void stage_terms(const StageArgs& a, long long idx, i
     const cplx<T>* raw = a.noise ? (const cplx<T>*)a.noise + (long long)b * 2 * nc * a.nk + idx : nullptr;
     cplx<T> w[D * D];
     // entry e of the on-device stream: w1 + a w2, or sqrt(1 + a^2) w with
-    // one draw when the step has a single noise stage
+    // one draw (the single-draw stream) when the step has a single noise stage
     const T sfac = fac * (T)sqrt(1.0 + a.noise_a * a.noise_a);
     auto draw = [&](int e) -> cplx<T> {
       cplx<T> w1, w2;
       if (a.single_draw) {
-        rng_draw<T>(seed, stp, idx, e >> 1, w1, w2);
-        return scl<T>((e & 1) ? w2 : w1, sfac);
+        if (zpre) return scl<T>(zpre[e], sfac);
+        rng_pair<T>(seed, stp, idx, e, w1, w2);
+        return scl<T>((stp & 1) ? w2 : w1, sfac);
       }
       rng_draw<T>(seed, stp, idx, e, w1, w2);
       return add<T>(scl<T>(w1, fac), scl<T>(scl<T>(w2, fac), na));
@@ -817,9 +839,25 @@ __global__ void __launch_bounds__(kThreads) stokes_steps_kernel(StageArgs a, int
 #pragma unroll
     for (int i = 0; i < D; ++i) u[i] = uh[i * a.nk];
     const long long st0 = a.msteps[b];
+    // the single-draw normals of a step pair, drawn at its first step here
+    constexpr int NE = D == 3 ? 3 : 2;
+    const int ne = a.noise_type == 0 ? (D == 3 ? 3 : 1) : D;
+    const unsigned long long seed = a.noise_on ? a.seeds[b] : 0ull;
+    cplx<T> ze[NE], zo[NE];
+#pragma unroll
+    for (int e = 0; e < NE; ++e) ze[e] = zo[e] = mk<T>(0, 0);
     for (int t = 0; t < nsteps; ++t) {
+      const long long stp = st0 + t;
+      if (a.noise_on && (t == 0 || (stp & 1) == 0)) {
+#pragma unroll
+        for (int e = 0; e < NE; ++e)
+          if (e < ne) rng_pair<T>(seed, stp, idx, e, ze[e], zo[e]);
+      }
+      cplx<T> zs[NE];
+#pragma unroll
+      for (int e = 0; e < NE; ++e) zs[e] = (stp & 1) ? zo[e] : ze[e];
       cplx<T> nl[D], nz[D], fr[D], r[D];
-      stage_terms<T, D>(a, idx, b, nullptr, u, m, st0 + t, nl, nz, fr);
+      stage_terms<T, D>(a, idx, b, nullptr, u, m, stp, nl, nz, fr, zs);
       stage_combine<T, D>(a, u, u, nl, nz, fr, Lk, r);
 #pragma unroll
       for (int i = 0; i < D; ++i) {

@@ -284,8 +284,8 @@ def test_anisotropic_box_rk3_thermal(mods, prec):
 def _single_draw_raw(seed, step, nk, shape, d, ntype, k):
     """The oracle's raw draws (w1, w2) that reproduce one step of the device's
     single-draw stream (implicit PC Stokes, whose only live noise stage has
-    a = 1): entry e of the stream is word pair e % 2 of counter group e // 2
-    (po.device_noise_draws), scaled by sqrt(1 + a^2) = sqrt 2.
+    a = 1): entry e of the stream (po.device_single_draws: word pair step % 2
+    of counter group e at step pair step // 2), scaled by sqrt(1 + a^2) = sqrt 2.
 
     Thermal: the device draws z (3 entries in 3D, one in 2D, along the third
     axis) and forms the projected noise as i k x z.  The oracle forms
@@ -294,9 +294,8 @@ def _single_draw_raw(seed, step, nk, shape, d, ntype, k):
     with v = sqrt 2 (k x z) gives W_sym k = v, which P leaves as it is
     (v is orthogonal to k); w1 = 0 at k = 0.  Correlated: w1 = sqrt 2 g."""
     ne = (3 if d == 3 else 1) if ntype == "thermal" else d
-    groups = (ne + 1) // 2
-    dev = po.device_noise_draws([seed], [step], nk, groups)[0]  # (2, groups, nk)
-    g = [dev[e % 2, e // 2].reshape(shape) for e in range(ne)]
+    dev = po.device_single_draws(seed, step, nk, ne)  # (ne, nk)
+    g = [dev[e].reshape(shape) for e in range(ne)]
     if ntype == "thermal":
         kk = [np.broadcast_to(k[i], shape) for i in range(d)]
         if d == 3:

@@ -334,3 +334,21 @@ def test_device_draw_layout_is_unit_normal():
     assert abs(np.corrcoef(d[0, 0, 0].real, d[0, 0, 1].real)[0, 1]) < 0.05
     e = po.device_noise_draws([7], [1], 4096, 1)
     assert not np.allclose(e[0, 0, 0], d[0, 0, 0])
+
+
+def test_device_single_draw_stream_is_unit_normal():
+    """The restated single-draw stream: unit variance per real part; the two
+    steps one call serves, later steps, entries and seeds uncorrelated; and
+    no draw shared with the multi-stage stream of the same seed."""
+    ev = po.device_single_draws(7, 10, 8192, 3)
+    od = po.device_single_draws(7, 11, 8192, 3)
+    nx = po.device_single_draws(7, 12, 8192, 3)
+    other = po.device_single_draws(8, 10, 8192, 3)
+    x = np.concatenate([ev.real.ravel(), ev.imag.ravel(), od.real.ravel(), od.imag.ravel()])
+    assert abs(x.mean()) < 0.02 and abs(x.var() - 1) < 0.03
+    for a, b in ((ev[0], od[0]), (ev[0], nx[0]), (ev[0], ev[1]), (ev[1], ev[2]), (ev[0], other[0]),
+                 (ev[0].real, ev[0].imag)):
+        assert abs(np.corrcoef(np.real(a), np.real(b))[0, 1]) < 0.05
+    # step pair 5 of the single-draw stream against step 5 of the multi-stage one
+    multi = po.device_noise_draws([7], [5], 8192, 3)[0].reshape(6, 8192)
+    assert not np.any(np.isclose(ev[:, None, :], multi[None, :, :]))
