@@ -525,11 +525,16 @@ template <typename T, int D> __device__ __forceinline__ void project(cplx<T>* v,
 // forcing.  nlin: the unprojected nonlinear term when the caller holds it,
 // else it is read from a.rhs_nl (or zero without a nonlinear term).  zpre:
 // this step's single-draw normals when the caller drew them (rng_pair), else
-// they are drawn here.
-template <typename T, int D>
+// they are drawn here.  RES: the register-resident Stokes steps (1 thermal,
+// 2 correlated noise), which have no nonlinear term, no injected draws and no
+// forcing, take their single-draw normals from zpre and the correlation
+// function's value from cpre: the branches they never take are compiled out
+// (the operations of the path they take are the same).
+template <typename T, int D, int RES = 0>
 __device__ __forceinline__ void stage_terms(const StageArgs& a, long long idx, int b, const cplx<T>* nlin,
                                             const cplx<T>* u, const Mode<T, D>& m, long long stp, cplx<T>* nl,
-                                            cplx<T>* nz, cplx<T>* fr, const cplx<T>* zpre = nullptr) {
+                                            cplx<T>* nz, cplx<T>* fr, const cplx<T>* zpre = nullptr,
+                                            const cplx<T>* cpre = nullptr) {
   const long long mb = (long long)b * D * a.nk;
 #pragma unroll
   for (int i = 0; i < D; ++i) {
@@ -538,7 +543,7 @@ __device__ __forceinline__ void stage_terms(const StageArgs& a, long long idx, i
     fr[i] = mk<T>(0, 0);
   }
   // nonlinear term: Leray projection (math.py:72-100) and optional Nyquist mask
-  if (nlin || a.rhs_nl) {
+  if (!RES && (nlin || a.rhs_nl)) {
     if (nlin) {
 #pragma unroll
       for (int i = 0; i < D; ++i) nl[i] = nlin[i];
@@ -558,15 +563,17 @@ __device__ __forceinline__ void stage_terms(const StageArgs& a, long long idx, i
   if (a.noise_on) {
     const T fac = m.k2 == (T)0 ? (T)0 : (T)a.noise_norm;
     const T na = (T)a.noise_a;
-    const unsigned long long seed = a.use_rng ? a.seeds[b] : 0ull;
-    const int nc = a.noise_type == 0 ? D * D : D;
-    const cplx<T>* raw = a.noise ? (const cplx<T>*)a.noise + (long long)b * 2 * nc * a.nk + idx : nullptr;
+    const bool thermal = RES ? RES == 1 : a.noise_type == 0;
+    const unsigned long long seed = !RES && a.use_rng ? a.seeds[b] : 0ull;
+    const int nc = thermal ? D * D : D;
+    const cplx<T>* raw = !RES && a.noise ? (const cplx<T>*)a.noise + (long long)b * 2 * nc * a.nk + idx : nullptr;
     cplx<T> w[D * D];
     // entry e of the on-device stream: w1 + a w2, or sqrt(1 + a^2) w with
     // one draw (the single-draw stream) when the step has a single noise stage
     const T sfac = fac * (T)sqrt(1.0 + a.noise_a * a.noise_a);
     auto draw = [&](int e) -> cplx<T> {
       cplx<T> w1, w2;
+      if (RES) return scl<T>(zpre[e], sfac);
       if (a.single_draw) {
         if (zpre) return scl<T>(zpre[e], sfac);
         rng_pair<T>(seed, stp, idx, e, w1, w2);
@@ -576,7 +583,7 @@ __device__ __forceinline__ void stage_terms(const StageArgs& a, long long idx, i
       return add<T>(scl<T>(w1, fac), scl<T>(scl<T>(w2, fac), na));
     };
     bool projected = false;  // nz already orthogonal to k
-    if (!raw && a.noise_type == 0) {
+    if (!raw && thermal) {
       // On-device stream, thermal noise.  The reference projects the
       // divergence of the symmetrised tensor, n = P (i W_sym k) (space_solver.py:
       // 240-265 with math.py:72-100).  W_sym has E W_ij W_lm* = s2 (d_il d_jm +
@@ -611,7 +618,7 @@ __device__ __forceinline__ void stage_terms(const StageArgs& a, long long idx, i
       }
     }
     if (!projected) {
-      if (a.noise_type == 0) {
+      if (thermal) {
         // math.py / space_solver.py:15-64: i k . (W + W^T) sqrt(1/2)
         const T h = (T)0.70710678118654752440;
 #pragma unroll
@@ -622,7 +629,7 @@ __device__ __forceinline__ void stage_terms(const StageArgs& a, long long idx, i
           nz[i] = mul_i<T>(acc);
         }
       } else {
-        const cplx<T> C = ((const cplx<T>*)a.corr)[idx];
+        const cplx<T> C = RES ? *cpre : ((const cplx<T>*)a.corr)[idx];
 #pragma unroll
         for (int i = 0; i < D; ++i) nz[i] = mul<T>(C, w[i]);
       }
@@ -634,7 +641,7 @@ __device__ __forceinline__ void stage_terms(const StageArgs& a, long long idx, i
   }
 
   // constant-power band forcing
-  if (a.fcoef && m.k2 >= (T)a.fk2min && m.k2 <= (T)a.fk2max) {
+  if (!RES && a.fcoef && m.k2 >= (T)a.fk2min && m.k2 <= (T)a.fk2max) {
     const T fc = (T)a.fcoef[b];
 #pragma unroll
     for (int i = 0; i < D; ++i) fr[i] = scl<T>(u[i], fc);
@@ -823,12 +830,15 @@ __global__ void __launch_bounds__(kThreads) stage_kernel(StageArgs a) {
 // arithmetic of stage_point for that step (bitwise the same state).  acc
 // (nullable) accumulates sum over the members, components and every step of
 // |u_hat|^2 in FP64 in registers, added to acc[idx] once at the end.
-template <typename T, int D, int MB>
+template <typename T, int D, int MB, int RES>
 __global__ void __launch_bounds__(kThreads) stokes_steps_kernel(StageArgs a, int nsteps, double* acc) {
+  static_assert(RES == 1 || RES == 2, "thermal (1) or correlated (2) noise");
   const long long idx = (long long)blockIdx.x * kThreads + threadIdx.x;
   if (idx >= a.nk) return;
   const Mode<T, D> m = decode<T, D>(a, idx);
   const T Lk = a.lin_op ? ((const T*)a.lin_op)[idx] : (T)(-a.nu) * m.k2;
+  // the correlation function at this mode (correlated noise)
+  const cplx<T> C = RES == 2 && a.noise_on ? ((const cplx<T>*)a.corr)[idx] : mk<T>(0, 0);
   const int b0 = blockIdx.y * MB;
   double s = 0.0;
   for (int bb = 0; bb < MB; ++bb) {
@@ -841,7 +851,7 @@ __global__ void __launch_bounds__(kThreads) stokes_steps_kernel(StageArgs a, int
     const long long st0 = a.msteps[b];
     // the single-draw normals of a step pair, drawn at its first step here
     constexpr int NE = D == 3 ? 3 : 2;
-    const int ne = a.noise_type == 0 ? (D == 3 ? 3 : 1) : D;
+    constexpr int ne = RES == 1 ? (D == 3 ? 3 : 1) : D;
     const unsigned long long seed = a.noise_on ? a.seeds[b] : 0ull;
     cplx<T> ze[NE], zo[NE];
 #pragma unroll
@@ -857,7 +867,7 @@ __global__ void __launch_bounds__(kThreads) stokes_steps_kernel(StageArgs a, int
 #pragma unroll
       for (int e = 0; e < NE; ++e) zs[e] = (stp & 1) ? zo[e] : ze[e];
       cplx<T> nl[D], nz[D], fr[D], r[D];
-      stage_terms<T, D>(a, idx, b, nullptr, u, m, stp, nl, nz, fr, zs);
+      stage_terms<T, D, RES>(a, idx, b, nullptr, u, m, stp, nl, nz, fr, zs, &C);
       stage_combine<T, D>(a, u, u, nl, nz, fr, Lk, r);
 #pragma unroll
       for (int i = 0; i < D; ++i) {

@@ -139,7 +139,14 @@ template <typename T> int launch_stokes_steps(int dim, const StageArgs& a, int n
   long long mb = blocks_x * a.n_members / want;
   const int MBv = mb >= 8 ? 8 : (mb >= 4 ? 4 : (mb >= 2 ? 2 : 1));
   const dim3 grid((unsigned)blocks_x, (unsigned)((a.n_members + MBv - 1) / MBv));
-#define PF_SK(D, M) hipLaunchKernelGGL((stokes_steps_kernel<T, D, M>), grid, dim3(kThreads), 0, s, a, nsteps, acc)
+  // the noise type as a compile-time choice (1 thermal, 2 correlated)
+#define PF_SK(D, M)                                                                                       \
+  do {                                                                                                   \
+    if (a.noise_type == 0)                                                                               \
+      hipLaunchKernelGGL((stokes_steps_kernel<T, D, M, 1>), grid, dim3(kThreads), 0, s, a, nsteps, acc); \
+    else                                                                                                 \
+      hipLaunchKernelGGL((stokes_steps_kernel<T, D, M, 2>), grid, dim3(kThreads), 0, s, a, nsteps, acc); \
+  } while (0)
   if (dim == 3) {
     switch (MBv) { case 8: PF_SK(3, 8); break; case 4: PF_SK(3, 4); break; case 2: PF_SK(3, 2); break; default: PF_SK(3, 1); }
   } else {
