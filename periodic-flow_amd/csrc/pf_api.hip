@@ -789,17 +789,19 @@ template <typename T> int issue_logs(LaunchLog* logs, int np, hipStream_t s, int
 }
 
 // n_steps steps of several plans, their nonlinear passes issued together
-// (issue_logs).  Plans that cannot go pass by pass together (other stage
-// counts or dimension, Stokes flow, no fused stage pass) step one after
-// another instead.  Noise is the on-device stream.
+// (issue_logs); a plan whose axis-0 forward pass is not fused with its stage
+// update runs its stage kernel after the issued passes.  Plans that cannot go
+// pass by pass together (other stage counts or dimension, Stokes flow) step
+// one after another instead.  Noise is the on-device stream.
 template <typename T> int step_multi_impl(pf_plan** ps, void** us, int np, long long n_steps, hipStream_t s,
                                           int* paired) {
   bool together = np >= 2;
+  bool fused[kMaxMulti] = {};
   for (int i = 0; i < np && together; ++i) {
     const pf_plan* p = ps[i];
-    const bool fused = p->fuse_stage &&
-                       tile_cols<T>(p->M[0], p->dim == 3 ? K_FWD_STAGE3 : K_FWD_STAGE2, p->tsel[PF_PASS_FWD0_STAGE]) > 0;
-    together = p->d.nonlinear && fused && p->d.integrator == ps[0]->d.integrator && p->dim == ps[0]->dim && !p->prof;
+    fused[i] = p->fuse_stage &&
+               tile_cols<T>(p->M[0], p->dim == 3 ? K_FWD_STAGE3 : K_FWD_STAGE2, p->tsel[PF_PASS_FWD0_STAGE]) > 0;
+    together = p->d.nonlinear && p->d.integrator == ps[0]->d.integrator && p->dim == ps[0]->dim && !p->prof;
   }
   if (!together) {
     for (int i = 0; i < np; ++i) {
@@ -812,6 +814,7 @@ template <typename T> int step_multi_impl(pf_plan** ps, void** us, int np, long 
   for (long long st = 0; st < n_steps; ++st) {
     for (int stage = 0; stage < ns; ++stage) {
       LaunchLog logs[kMaxMulti];
+      StageArgs sa[kMaxMulti];
       for (int i = 0; i < np; ++i) {
         pf_plan* p = ps[i];
         if (p->d.forcing == PF_FORCING_CONSTANT_POWER) {
@@ -819,11 +822,17 @@ template <typename T> int step_multi_impl(pf_plan** ps, void** us, int np, long 
           if (rc) return rc;
         }
         const bool noise_on = p->d.noise_mag != 0.0 && uses_noise(p->d.integrator);
-        StageArgs a = stage_args(p, us[i], stage, nullptr, noise_on);
-        a.rhs_nl = nullptr;
+        StageArgs& a = sa[i];
+        a = stage_args(p, us[i], stage, nullptr, noise_on);
         p->rec = &logs[i];
         int rc = nonlinear_front<T>(p, us[i], s);
-        if (!rc) rc = last_forward<T>(p, p->dim == 3 ? K_FWD_STAGE3 : K_FWD_STAGE2, nullptr, &a, s);
+        if (!rc && fused[i]) {
+          StageArgs f = a;
+          f.rhs_nl = nullptr;
+          rc = last_forward<T>(p, p->dim == 3 ? K_FWD_STAGE3 : K_FWD_STAGE2, nullptr, &f, s);
+        } else if (!rc) {
+          rc = last_forward<T>(p, K_FWD, p->w1, nullptr, s);  // R; the stage kernel follows
+        }
         p->rec = nullptr;
         if (rc) return rc;
         if (logs[i].overflow) return fail(PF_EINVAL, "step_multi: more launches per stage than recorded");
@@ -843,6 +852,8 @@ template <typename T> int step_multi_impl(pf_plan** ps, void** us, int np, long 
           if (rc) return rc;
         }
       }
+      for (int i = 0; i < np; ++i)
+        if (!fused[i]) PF_TRY(launch_stage<T>(ps[i]->dim, sa[i], s, stage_cache_policy(ps[i])));
     }
     for (int i = 0; i < np; ++i) {
       PF_TRY(launch_member_step(ps[i]->msteps, ps[i]->nact, s));

@@ -4,7 +4,8 @@ Members with the same shape and parameters share one plan (one launch per
 pass for all of them).  Groups of different shapes advance together through
 pf_step_multi: every stage issues each pass of all groups at once, and two
 groups whose transform lengths form a compiled pair (e.g. 128^3 and 96^3
-under the 3/2 rule: 192 and 144) share one launch of that pass.  A step can
+under the 3/2 rule: 192 and 144) share one launch of that pass, unless
+autotune() finds the groups faster stepped one after another.  A step can
 also be captured into one HIP graph and replayed.  Every member keeps its
 own seed and RNG step counter, so its result does not depend on the other
 members.  The reference runs one solve per MPI job; this layer has no
@@ -13,6 +14,7 @@ counterpart there.
 from __future__ import annotations
 
 import math
+import time
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -50,6 +52,7 @@ class Group:
     plan: Plan
     state: torch.Tensor
     members: list = field(default_factory=list)  # global member ids, slot order
+    seeds: list = field(default_factory=list)    # per slot
 
 
 class Ensemble:
@@ -81,11 +84,15 @@ class Ensemble:
             if spec.noise_type == "correlated" and spec.noise_mag != 0 and correlation is not None:
                 C = torch.as_tensor(np.asarray(correlation(spec)), device="cuda", dtype=plan.complex_dtype)
                 plan.set_correlation(C.contiguous())
-            g = Group(spec, plan, plan.new_state(), [i for i, _, _ in lst])
+            g = Group(spec, plan, plan.new_state(), [i for i, _, _ in lst], [s for _, _, s in lst])
             self.groups[k] = g
             for slot, (i, _, _) in enumerate(lst):
                 self.where[i] = (k, slot)
         self._graph = None
+        # groups of different grids step pass by pass together, with paired
+        # launches (pf_step_multi); autotune() keeps this only where it is
+        # not slower than stepping the groups one after another
+        self.pair = True
 
     def __len__(self):
         return len(self.where)
@@ -104,23 +111,52 @@ class Ensemble:
         return sum(int(np.prod(g.spec.n)) * g.plan.batch for g in self.groups.values())
 
     def autotune(self, reps=5):
-        """Choose every group's launch plans by timing (Plan.autotune); returns
-        {group key: {pass name: chosen ms}}.  A member then equals the same
-        member run alone bit for bit only under the same plan (the radix
-        factorisation changes rounding)."""
+        """Choose every group's launch plans by timing (Plan.autotune), then,
+        with several groups, paired launches against the groups stepped one
+        after another (``self.pair``: kept unless stepping apart is at least
+        3% faster), timed on the states themselves, which are then put back
+        with every member's RNG step counter.  Returns {group key: {pass
+        name: chosen ms}}.  A member then equals the same member run alone
+        bit for bit only under the same plan (the radix factorisation changes
+        rounding)."""
         self._tune = False
         self._graph = None
-        return {k: g.plan.autotune(g.state, reps) for k, g in self.groups.items()}
+        out = {k: g.plan.autotune(g.state, reps) for k, g in self.groups.items()}
+        gs = list(self.groups.values())
+        if len(gs) > 1 and all(g.spec.nonlinear for g in gs):
+            saved = [(g.state.clone(), g.plan.step_counter, g.plan.member_steps()) for g in gs]
+            self.pair = True
+            self._step(1)
+            t = {}
+            for mode in (True, False, True, False):
+                self.pair = mode
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                self._step(2)
+                torch.cuda.synchronize()
+                dt = time.perf_counter() - t0
+                t[mode] = min(t.get(mode, dt), dt)
+            self.pair = not t[False] < 0.97 * t[True]
+            for g, (st, host, msteps) in zip(gs, saved):
+                g.state.copy_(st)
+                g.plan.step_counter = host
+                for slot, (seed, m) in enumerate(zip(g.seeds, msteps)):
+                    g.plan.set_member(slot, seed, m)
+        return out
+
+    def _step(self, n_steps):
+        gs = list(self.groups.values())
+        if len(gs) == 1 or not self.pair:
+            for g in gs:
+                g.plan.step(g.state, n_steps)
+            return 0
+        return step_multi([g.plan for g in gs], [g.state for g in gs], n_steps)
 
     def step(self, n_steps=1):
         """Advance every member n_steps; returns the number of paired launches."""
         if self._tune:
             self.autotune()
-        gs = list(self.groups.values())
-        if len(gs) == 1:
-            gs[0].plan.step(gs[0].state, n_steps)
-            return 0
-        return step_multi([g.plan for g in gs], [g.state for g in gs], n_steps)
+        return self._step(n_steps)
 
     # ---------------------------------------------------------------- graphs
     def capture(self):

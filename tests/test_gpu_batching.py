@@ -229,12 +229,47 @@ def test_paired_launches_equal_separate_plans(mods, prec):
         assert ens.plan_of(i).step_counter == 3
 
 
-def test_paired_launches_cache_policy(mods):
+def test_ensemble_autotune_leaves_members_and_counters(mods):
+    """Ensemble.autotune times paired against separate stepping on the states
+    themselves: afterwards every member and RNG step counter is as before,
+    and a step equals each member's plan stepped alone (same plan)."""
+    ensemble, plan, _ = mods
+    _say("ensemble autotune restores states")
+    S = ensemble.MemberSpec
+    members = [(S(n=(32,) * 3, dt=5e-3, integrator="rk3", viscosity=0.01, noise_mag=0.01), 41),
+               (S(n=(24,) * 3, dt=5e-3, integrator="rk3", viscosity=0.01, noise_mag=0.01), 42)]
+    ens = ensemble.Ensemble(members, precision="fp32")
+    for i in range(2):
+        ens.set_fourier(i, ensemble.random_solenoidal(ens.plan_of(i), 60 + i, energy=0.5))
+    ens.step(1)
+    before = [ens.state(i).clone() for i in range(2)]
+    counters = [(ens.plan_of(i).step_counter, ens.plan_of(i).member_steps()) for i in range(2)]
+    ens.autotune(reps=2)
+    assert ens.pair in (True, False)
+    for i in range(2):
+        assert torch.equal(ens.state(i), before[i]), i
+        assert (ens.plan_of(i).step_counter, ens.plan_of(i).member_steps()) == counters[i]
+    ens.step(2)
+    for i, (spec, seed) in enumerate(members):
+        tiles, fused = ens.plan_of(i).tiles
+        p = plan.Plan(spec.n, spec.dt, precision="fp32", integrator="rk3", viscosity=spec.viscosity,
+                      noise_mag=spec.noise_mag, seeds=[seed])
+        p.set_tiles(tiles)
+        p.set_fusion(fused)
+        p.step_counter = 1
+        st = before[i][None].clone().contiguous()
+        p.step(st, 2)
+        assert torch.equal(st[0], ens.state(i)), i
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_paired_launches_cache_policy(mods, fused):
     """Paired launches at the nontemporal launch variants (both groups at the
     same variant of every pass): still one paired launch per pass, and the
-    members equal each plan stepped alone at the default variants."""
+    members equal each plan stepped alone at the default variants.  Unfused:
+    the axis-0 forward passes pair and each group's stage kernel follows."""
     ensemble, plan, _ = mods
-    _say("paired launches, nontemporal variants")
+    _say("paired launches, nontemporal variants, fused", fused)
     S = ensemble.MemberSpec
     members = [(S(n=(128,) * 3, dt=5e-3, integrator="rk3", viscosity=0.005, noise_mag=0.01), 31),
                (S(n=(96,) * 3, dt=5e-3, integrator="rk3", viscosity=0.005, noise_mag=0.01), 32)]
@@ -244,14 +279,17 @@ def test_paired_launches_cache_policy(mods):
         u = ensemble.random_solenoidal(ens.plan_of(i), 50 + i, energy=0.5)
         inits.append(u)
         ens.set_fourier(i, u)
-    nt = {"inv_axis0_curl": 6, "inv_axis1": 7, "row_c2r_cross_r2c": 4, "fwd_axis1": 6, "fwd_axis0_stage": 6}
+    nt = {"inv_axis0_curl": 6, "inv_axis1": 7, "row_c2r_cross_r2c": 4, "fwd_axis1": 6, "fwd_axis0_stage": 6,
+          "fwd_axis0": 7, "stage": 7}
     for g in ens.groups.values():
         g.plan.set_tiles(nt)
+        g.plan.set_fusion(fused)
     paired = ens.step(2)
     assert paired == 30, paired  # 2 steps x 3 stages x 5 passes
     for i, (spec, seed) in enumerate(members):
         p = plan.Plan(spec.n, spec.dt, precision="fp32", integrator="rk3", viscosity=spec.viscosity,
                       noise_mag=spec.noise_mag, seeds=[seed])
+        p.set_fusion(fused)
         st = torch.as_tensor(inits[i][None], device="cuda", dtype=p.complex_dtype).contiguous()
         p.step(st, 2)
         assert torch.equal(st[0], ens.state(i)), i
