@@ -290,9 +290,10 @@ def main(argv=None):
     python -m periodicflow.workloads stokes512 --steps 600
         thermal-noise Stokes 512^2 x 256 realisations, register-resident
         steps with the FP64 structure factor accumulated
-    python -m periodicflow.workloads forced --steps 5
+    python -m periodicflow.workloads forced --steps 5 [--plan PLAN]
         the forced-turbulence ensemble 16 x 128^3 + 16 x 96^3, default plans
-        (no autotuning, so the trace holds only the steps)
+        or the given ones (no autotuning, so the trace holds only the steps);
+        PLAN: 'N:pass=variant,...,fused=0|1;N:...' per group of grid size N
     python -m periodicflow.workloads tgv256 --steps 5
         Taylor-Green 256^3, RK4, 2/3 rule, FP32
 
@@ -304,6 +305,7 @@ def main(argv=None):
     ap.add_argument("workload", choices=["stokes512", "forced", "tgv256"])
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--calibrate-copy", action="store_true")
+    ap.add_argument("--plan", default=None, help="forced: launch plans per group, 'N:pass=v,...,fused=0|1;N:...'")
     args = ap.parse_args(argv)
     if args.workload == "stokes512":
         n, nu, dt, kT, B = 512, 1.0, 0.05, 1e-3, 256
@@ -319,6 +321,17 @@ def main(argv=None):
         ens = Ensemble(members, precision="fp32", correlation=forced_correlation)
         for i, (spec, seed) in enumerate(members):
             ens.set_fourier(i, random_solenoidal(ens.plan_of(i), seed, energy=0.5, kp=4.0))
+        for part in (args.plan or "").split(";"):
+            if not part:
+                continue
+            size, kvs = part.split(":")
+            kv = dict(item.split("=") for item in kvs.split(",") if item)
+            fused = kv.pop("fused", None)
+            for g in ens.groups.values():
+                if g.spec.n[0] == int(size):
+                    g.plan.set_tiles({k: int(v) for k, v in kv.items()})
+                    if fused is not None:
+                        g.plan.set_fusion(bool(int(fused)))
         t = _timed(lambda: ens.step(args.steps))
         rate = ens.points_per_step() * args.steps / t / 1e6
     else:
