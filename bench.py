@@ -105,7 +105,9 @@ def batched_vs_plain(torch, members=256, steps=100, plain_members=32):
     """Batched FP32 ensemble stepping vs the plain path (the same kernels at
     batch 1, one solve after another), on the 2D thermal-noise Stokes
     ensemble (512^2, implicit PC, dt = 0.05, nu = 1, kT = 1e-3, seeds
-    1000 + r) and on a small 3D forced-turbulence ensemble (64^3, RK3)."""
+    1000 + r), on a small 3D forced-turbulence ensemble (64^3, RK3), on the
+    forced 16 x 128^3 + 16 x 96^3 ensemble, and on members too small to fill
+    the GPU alone (small_batched_vs_plain)."""
     import math
 
     from periodicflow.ensemble import random_solenoidal
@@ -169,6 +171,87 @@ def batched_vs_plain(torch, members=256, steps=100, plain_members=32):
                               "plain_members_timed": 8, "speedup": round(rb / rp, 2)}
     del pb, ub, pp, up
     out["forced3d_16x128_16x96_rk3"] = forced_batched_vs_plain(torch)
+    out.update(small_batched_vs_plain(torch))
+    return out
+
+
+def small_batched_vs_plain(torch):
+    """Members too small to fill the GPU alone, the regime batching is for
+    (the request stream's 128^2 and 256^2 grids): thermal-noise Stokes 128^2
+    (implicit PC, dt = 0.05, nu = 1, kT = 1e-3) x 2048 realisations, and
+    nonlinear 2D flow 256^2 (3/2 rule, RK3, nu = 1e-3, dt = 1e-3, seeded
+    solenoidal starts) x 128 members; each against the same members one at a
+    time, with member 0's FP32 state against FP64 after the timed steps."""
+    import math
+
+    from periodicflow.ensemble import random_solenoidal
+    from periodicflow.plan import Plan
+
+    out = {}
+
+    def rel(a, b):
+        return float(torch.linalg.vector_norm(a.to(torch.complex128) - b) / torch.linalg.vector_norm(b))
+
+    # thermal-noise Stokes 128^2
+    n, nu, dt, kT, B, steps, pm = 128, 1.0, 0.05, 1e-3, 2048, 200, 16
+    nm = math.sqrt(2 * nu * kT / (2 * math.pi / n) ** 2)
+    kw = dict(integrator="implicit_pc", nonlinear=False, viscosity=nu, noise_mag=nm)
+    pb = Plan((n, n), dt, precision="fp32", batch=B, seeds=[1000 + r for r in range(B)], **kw)
+    ub = pb.new_state()
+    pb.step(ub, 2)
+    ub.zero_()
+    pb.step_counter = 0
+    rb = _rate(torch, lambda: pb.step(ub, steps), n * n * B * steps)
+    pp = Plan((n, n), dt, precision="fp32", batch=1, **kw)
+    up = pp.new_state()
+    pp.step(up, 2)
+
+    def plain():
+        for m in range(pm):
+            pp.set_member(0, 1000 + m, 0)
+            up.zero_()
+            pp.step(up, steps)
+
+    rp = _rate(torch, plain, n * n * pm * steps)
+    p64 = Plan((n, n), dt, precision="fp64", batch=1, seeds=[1000], **kw)
+    u64 = p64.new_state()
+    p64.step(u64, steps)
+    out["stokes2d_128_thermal_implicit_pc"] = {
+        "members": B, "steps": steps, "batched": round(rb, 1), "plain": round(rp, 1), "plain_members_timed": pm,
+        "speedup": round(rb / rp, 2), "fp32_rel_l2_vs_fp64": rel(ub[0], u64[0])}
+    del pb, ub, pp, up, p64, u64
+
+    # nonlinear 2D 256^2
+    n, B, steps, pm = 256, 128, 50, 8
+    kw = dict(integrator="rk3", viscosity=1e-3, dealias="3/2")
+    pb = Plan((n, n), 1e-3, precision="fp32", batch=B, **kw)
+    ub = pb.new_state()
+    inits = [random_solenoidal(pb, 500 + m, energy=0.5, kp=8.0) for m in range(B)]
+    for m in range(B):
+        ub[m].copy_(torch.as_tensor(inits[m], device="cuda", dtype=pb.complex_dtype))
+    u_start = ub.clone()
+    pb.autotune(ub)
+    pb.step(ub, 1)
+    ub.copy_(u_start)
+    rb = _rate(torch, lambda: pb.step(ub, steps), n * n * B * steps)
+    pp = Plan((n, n), 1e-3, precision="fp32", batch=1, **kw)
+    up = pp.new_state()
+    up.copy_(u_start[0:1])
+    pp.autotune(up)
+    pp.step(up, 1)
+
+    def plain2():
+        for m in range(pm):
+            up.copy_(u_start[m:m + 1])
+            pp.step(up, steps)
+
+    rp = _rate(torch, plain2, n * n * pm * steps)
+    p64 = Plan((n, n), 1e-3, precision="fp64", batch=1, **kw)
+    u64 = torch.as_tensor(inits[0][None], device="cuda", dtype=p64.complex_dtype).contiguous()
+    p64.step(u64, steps)
+    out["nonlinear2d_256_rk3"] = {
+        "members": B, "steps": steps, "batched": round(rb, 1), "plain": round(rp, 1), "plain_members_timed": pm,
+        "speedup": round(rb / rp, 2), "fp32_rel_l2_vs_fp64": rel(ub[0], u64[0])}
     return out
 
 
