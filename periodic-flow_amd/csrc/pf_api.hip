@@ -1012,10 +1012,12 @@ template <typename T> int autotune_impl(pf_plan* p, const void* uhat, int reps, 
     stage_ms = stage_def;
   }
   p->tsel[PF_PASS_STAGE] = stage_cp ? 7 : 0;
-  // fused against unfused axis-0 forward pass + stage update (fused unless
-  // the unfused pair is clearly faster)
+  // fused against unfused axis-0 forward pass + stage update: fused only
+  // where it is clearly faster.  The unfused stage kernel streams the state
+  // and keeps its time from box to box, where the fused pass's strided stage
+  // traffic varies by several percent, so a tie goes to the unfused pair.
   if (best[PF_PASS_FWD0_STAGE] >= 0.0 && best[PF_PASS_FWD0] >= 0.0 && stage_ms >= 0.0)
-    p->fuse_stage = best[PF_PASS_FWD0] + stage_ms > kMargin * best[PF_PASS_FWD0_STAGE];
+    p->fuse_stage = best[PF_PASS_FWD0_STAGE] < kMargin * (best[PF_PASS_FWD0] + stage_ms);
   if (ms_out) {
     for (int id : tunable) ms_out[id] = best[id] < 0.0 ? 0.0 : best[id];
     ms_out[PF_PASS_STAGE] = stage_ms < 0.0 ? 0.0 : stage_ms;
