@@ -3,15 +3,15 @@
 // HBM layout (C order, complex interleaved, one block per member):
 //   state u_hat         (B, d, N0, [N1,] nh)      nh = N_last/2 + 1
 //   3D nonlinear passes (per member, F = 6 for u_hat and omega_hat; rows of
-//   A, B, C padded to the pitch P >= nh, a multiple of 128 bytes):
+//   A, B, C, D padded to the pitch P >= nh, a multiple of 128 bytes):
 //     A  (F, M0, N1, P)    axis-0 inverse, padded       <- col_pass UC3
 //     B  (F, M0, M1, P)    axis-1 inverse, padded       <- col_pass
 //     C  (3, M0, M1, P)    c2r, u x omega, r2c, cut     <- row_pass FUSED3
-//     D  (3, M0, N1, nh)   axis-1 forward, truncated    <- col_pass (planes 128 B apart)
+//     D  (3, M0, N1, P)    axis-1 forward, truncated    <- col_pass (planes 128 B apart)
 //     R  (3, N0, N1, nh)   axis-0 forward, truncated    <- col_pass (or fused stage)
 //   2D: A (3, M0, P) -> C (2, M0, P) -> R (2, N0, nh)
-// D has the state's layout so that the axis-0 forward pass walks one flat
-// column index over both.
+// The axis-0 forward pass enumerates D's N1 rows of nh entries back to back
+// and maps each column to D's row (pitch P) and to the state's (pitch nh).
 // The padded physical fields never reach HBM: the last axis is taken to
 // physical space, multiplied and brought back inside one block.
 #pragma once
