@@ -340,13 +340,14 @@ template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStrea
     const long long D_f = (long long)M0 * p->dplane;  // D: rows of pitch P
     // retained last-axis entries (2/3 rule) or all nh of them (3/2 rule)
     const int lval = pr ? N2 / 3 + 1 : (int)nh;
-    // P1: axis-0 inverse of u_hat and omega_hat, u_hat -> A (6 fields, w1)
+    // P1: axis-0 inverse of u_hat and of omega_y, omega_z, u_hat -> A
+    // (5 fields, w1)
     ColArgs a = col_args();
     a.in = uhat; a.out = p->w1;
     a.rows_in = N0; a.rows_out = M0;
     a.in_row = (long long)N1 * nh; a.out_row = p->aplane;
     a.in_field = p->nk; a.out_field = A_f;
-    a.in_member = 3 * p->nk; a.out_member = 6 * A_f;
+    a.in_member = 3 * p->nk; a.out_member = 5 * A_f;
     a.n_fields = 3; a.n_members = B;
     a.gN0 = N0; a.gN1 = N1; a.gN2 = N2; a.gnh = (int)nh;
     a.ks0 = p->ks[0]; a.ks1 = p->ks[1]; a.ks2 = p->ks[2];
@@ -364,24 +365,28 @@ template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStrea
       // The (j, l) columns are enumerated back to back, so no tile is
       // padded at a group's end (3/2 rule at 96^3: 49 entries per group,
       // which whole tiles per group rounded up to 64; 15% faster).
-      // one launch: blocks of fields 0..2 transform u_hat, blocks of 3..5 omega_hat.
+      // one launch: blocks of fields 0..2 transform u_hat, blocks of 3, 4
+      // omega_y and omega_z (omega_x = i (ky w - kz v) needs no axis-0
+      // derivative: P2 forms it at its loads, so A holds 5 fields, not 6).
       // With nontemporal stores (whose partial lines cost a read-modify-write
       // of the line) each group starts a new tile and whole tiles are stored
       // into A's padded rows, so every store is whole 128-byte lines.
       a.f_base = 0;
-      a.n_fields = 6;
+      a.n_fields = 5;
       const int v0 = p->tsel[PF_PASS_INV0];
       const bool lines = col_cache_policy(v0) != 0;
       PF_TRY(col_groups<T>(p, M0, K_INV_UC3, v0, a, ngroups, nh, P, lval, s, 0, lines ? (int)P : 0, !lines));
     }
-    // P2: axis-1 inverse, A -> Bbuf (w2)
+    // P2: axis-1 inverse, A -> Bbuf (w2), velocity and vorticity, omega_x
+    // formed from v and w at the loads
     a = col_args();
     a.in = p->w1; a.out = p->w2;
     a.rows_in = N1; a.rows_out = M1;
     a.in_row = P; a.out_row = P;
     a.in_field = A_f; a.out_field = B_f;
-    a.in_member = 6 * A_f; a.out_member = 6 * B_f;
+    a.in_member = 5 * A_f; a.out_member = 6 * B_f;
     a.n_fields = 6; a.n_members = B;
+    a.ks1 = p->ks[1]; a.ks2 = p->ks[2];
     if (pr) {
       a.prune = 1;
       a.Krow = N1 / 3;
@@ -394,7 +399,7 @@ template <typename T> int nonlinear_front(pf_plan* p, const void* uhat, hipStrea
       // their own columns, and B's entries past lval are not read by the
       // last-axis pass.
       const int cap = (int)P;
-      PF_TRY(col_planes<T>(p, M1, K_INV, p->tsel[PF_PASS_INV1], a, M0, p->aplane, (long long)M1 * P, lval,
+      PF_TRY(col_planes<T>(p, M1, K_INV_CURL3, p->tsel[PF_PASS_INV1], a, M0, p->aplane, (long long)M1 * P, lval,
                            s, cap, cap));
     }
     // P3: last axis c2r, u x omega, r2c, truncate: Bbuf -> C (w1)
@@ -895,7 +900,7 @@ template <typename T> bool variant_ok(const pf_plan* p, int pass, int v) {
   switch (pass) {
     case PF_PASS_INV0:
       return tile_cols<T>(p->M[0], d3 ? K_INV_UC3 : K_INV_UC2, v) > 0;
-    case PF_PASS_INV1: return d3 && tile_cols<T>(p->M[1], K_INV, v) > 0;
+    case PF_PASS_INV1: return d3 && tile_cols<T>(p->M[1], K_INV_CURL3, v) > 0;
     case PF_PASS_ROW: return row_pair_count<T>(p->M[p->dim - 1], d3 ? ROW_FUSED3 : ROW_FUSED2, v) > 0;
     case PF_PASS_FWD1: return d3 && tile_cols<T>(p->M[1], K_FWD, v) > 0;
     case PF_PASS_FWD0: return tile_cols<T>(p->M[0], K_FWD, v) > 0;

@@ -2,10 +2,10 @@
 //
 // HBM layout (C order, complex interleaved, one block per member):
 //   state u_hat         (B, d, N0, [N1,] nh)      nh = N_last/2 + 1
-//   3D nonlinear passes (per member, F = 6 for u_hat and omega_hat; rows of
-//   A, B, C, D padded to the pitch P >= nh, a multiple of 128 bytes):
-//     A  (F, M0, N1, P)    axis-0 inverse, padded       <- col_pass UC3
-//     B  (F, M0, M1, P)    axis-1 inverse, padded       <- col_pass
+//   3D nonlinear passes (per member; rows of A, B, C, D padded to the pitch
+//   P >= nh, a multiple of 128 bytes):
+//     A  (5, M0, N1, P)    axis-0 inverse of u_hat, omega_y, omega_z         <- col_pass UC3
+//     B  (6, M0, M1, P)    axis-1 inverse, velocity and vorticity           <- col_pass CURL3
 //     C  (3, M0, M1, P)    c2r, u x omega, r2c, cut     <- row_pass FUSED3
 //     D  (3, M0, N1, P)    axis-1 forward, truncated    <- col_pass (planes 128 B apart)
 //     R  (3, N0, N1, nh)   axis-0 forward, truncated    <- col_pass (or fused stage)
@@ -1053,7 +1053,9 @@ __global__ void __launch_bounds__(kThreads) spectrum_kernel(DiagArgs a, int nshe
 // columns into LDS (coalesced row segments), transforms every column and
 // writes the tile back.  Kinds:
 //   K_INV        inverse, rows_in -> L with 3/2 zero padding (Nyquist split)
-//   K_INV_UC3    axis 0 of 3D: emits u_hat and omega_hat = i k x u_hat
+//   K_INV_UC3    axis 0 of 3D: emits u_hat and omega_y, omega_z of omega_hat = i k x u_hat
+//   K_INV_CURL3  axis 1 of 3D from K_INV_UC3's five fields: emits u, omega_x =
+//                i (ky w - kz v) formed at the load (ky, kz are diagonal there), omega_y, omega_z
 //   K_INV_UC2    axis 0 of 2D: emits u_hat and the scalar vorticity
 //   K_FWD        forward, L -> rows_out with truncation (Nyquist sum)
 //   K_FWD_STAGE3/2  forward axis 0 of all components of a member, then the
@@ -1067,7 +1069,8 @@ enum ColKind {
   K_FWD = 3,
   K_FWD_STAGE3 = 4,
   K_FWD_STAGE2 = 5,
-  K_INV_UC3 = 8,     // u_hat -> velocity (fg 0..2) and vorticity (fg 3..5) of a 3D field
+  K_INV_UC3 = 8,     // u_hat -> velocity (fg 0..2) and omega_y, omega_z (fg 3, 4) of a 3D field
+  K_INV_CURL3 = 10,  // K_INV_UC3's fields -> velocity (fg 0..2) and vorticity (fg 3..5)
   K_INV_UC2 = 9,     // u_hat -> velocity (fg 0..1) and scalar vorticity (fg 2) of a 2D field
 };
 
@@ -1170,8 +1173,8 @@ __device__ __forceinline__ long long xcd_order(long long bid, long long G) {
   return x * q + (x < r ? x : r) + bid / 8;
 }
 
-// a bool as a type, to select a lambda's code path at compile time
-template <bool B> struct BoolC { static constexpr bool value = B; };
+// an int as a type, to select a lambda's code path at compile time
+template <int I> struct IntC { static constexpr int value = I; };
 
 template <int KIND> constexpr int col_fields() {
   return KIND == K_FWD_STAGE3 ? 3 : (KIND == K_FWD_STAGE2 ? 2 : 1);
@@ -1186,7 +1189,8 @@ template <typename T, int L, int TC, int KIND, int PIPE, int NT, int RP = 0, int
 __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __restrict__ tw, cplx<T>* tws,
                                          cplx<T>* buf, long long bid, long long nblk) {
   constexpr bool STAGE = KIND == K_FWD_STAGE3 || KIND == K_FWD_STAGE2;
-  constexpr bool INV = KIND == K_INV || KIND == K_INV_UC3 || KIND == K_INV_UC2;
+  constexpr bool CURL = KIND == K_INV_CURL3;
+  constexpr bool INV = KIND == K_INV || KIND == K_INV_UC3 || KIND == K_INV_UC2 || CURL;
   constexpr bool UC = KIND == K_INV_UC3 || KIND == K_INV_UC2;
   constexpr bool GRID = UC;  // loads need the mode numbers of the unpadded grid
   constexpr int GD = KIND == K_INV_UC3 ? 3 : 2;
@@ -1209,7 +1213,7 @@ __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __rest
   // find them in the same XCD's L2
   const int ntb = (gm.n_tiles + PIPE - 1) / PIPE;
   int tb, fg;
-  if constexpr (UC) {
+  if constexpr (UC || CURL) {
     fg = (int)(lg % nfg);
     lg /= nfg;
     tb = (int)(lg % ntb);
@@ -1220,9 +1224,12 @@ __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __rest
     fg = (int)(lg % nfg);
     lg /= nfg;
   }
-  // velocity or vorticity block (uniform over the block) and its component
-  const bool curlb = UC && fg >= GD;
-  const int cfg = (UC && fg >= GD) ? fg - GD : fg;
+  // The block's role (uniform over the block): a curl component formed from
+  // two loaded fields (curlb: the 2D curl, omega_y and omega_z of
+  // K_INV_UC3, omega_x of K_INV_CURL3) or one loaded field; cfg: the curl
+  // component or the field loaded
+  const bool curlb = (UC && fg >= GD) || (CURL && fg == 3);
+  const int cfg = KIND == K_INV_UC2 && fg >= 2 ? fg - 2 : (KIND == K_INV_UC3 && fg >= 3 ? fg - 2 : (CURL && fg >= 4 ? fg - 1 : fg));
   const int outer = (int)(lg % gm.n_outer);
   const int b = (int)(lg / gm.n_outer);
   int c0 = tb * PIPE * TC;  // first column of the tile in hand
@@ -1280,12 +1287,19 @@ __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __rest
     else if (f == 0) { ia = 2; ib = 1; ka = k1; kb = k2; }
     else { ia = 0; ib = 2; ka = k2; kb = k0; }
   };
+  // K_INV_CURL3's omega_x = i (ky w - kz v) at padded row src (the axis-1
+  // mode) and column l (the last-axis mode): inputs w (field 2) and v (1)
+  auto vort_k = [&](int src, int l, T& ka, T& kb) {
+    ka = (T)(cmode(src, a.rows_in) * a.ks1);
+    kb = (T)(l * a.ks2);
+  };
   // Loads of this thread's LDS slots base .. base + NQ - 1 of the tile at c0
   // (NQ = the register arrays' length), all issued before any is used.
-  // CB: a vorticity block (a compile-time copy of curlb, so that the
+  // ROLE: 1 for a curl block (a compile-time copy of curlb, so that the
   // unrolled loops hold no per-entry branch between their loads)
-  auto issue = [&](int base, auto& ra, auto& rb, auto cb) {
-    constexpr bool CB = decltype(cb)::value;
+  auto issue = [&](int base, auto& ra, auto& rb, auto rl) {
+    constexpr int ROLE = decltype(rl)::value;
+    constexpr bool CB = ROLE == 1;
     constexpr int NQ = (int)(sizeof(ra) / sizeof(ra[0]));
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
@@ -1301,9 +1315,17 @@ __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __rest
       const long long col = (long long)j * gm.in_lp + l;
       double fc;
       int src = INV ? pad_src(e, a.rows_in, L, fc) : e;
-      if (KIND == K_INV && a.prune && src >= 0 && abs(cmode(src, a.rows_in)) > a.Krow) src = -1;
+      if ((KIND == K_INV || CURL) && a.prune && src >= 0 && abs(cmode(src, a.rows_in)) > a.Krow) src = -1;
       if (src < 0) continue;
-      if constexpr (GRID) {
+      if constexpr (CURL) {
+        const cplx<T>* p = in + (long long)src * a.in_row + col;
+        if constexpr (!CB) {
+          ra[q] = ld_stream<cp_load(CP), T>(p + (long long)cfg * a.in_field);
+        } else {
+          ra[q] = ld_stream<cp_load(CP), T>(p + 2 * a.in_field);
+          rb[q] = ld_stream<cp_load(CP), T>(p + a.in_field);
+        }
+      } else if constexpr (GRID) {
         const int mi = cmode(src, a.gN0);
         const int mj = GD == 3 ? cmode(j, a.gN1) : 0;
         if (a.mask23 && (abs(mi) > a.gN0 / 3 || abs(mj) > a.gN1 / 3 || l > (GD == 3 ? a.gN2 : a.gN1) / 3))
@@ -1324,23 +1346,25 @@ __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __rest
     }
   };
   // ... combined (curl, 3/2-rule Nyquist halving) and stored to LDS
-  auto commit = [&](int base, const auto& ra, const auto& rb, auto cb) {
-    constexpr bool CB = decltype(cb)::value;
+  auto commit = [&](int base, const auto& ra, const auto& rb, auto rl) {
+    constexpr int ROLE = decltype(rl)::value;
     constexpr int NQ = (int)(sizeof(ra) / sizeof(ra[0]));
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const int idx = threadIdx.x + (base + q) * NT;
       if (base + q >= NIT || idx >= L * W) continue;
       cplx<T> v = ra[q];
-      if constexpr (CB) {
-        // entries not loaded are zero in ra and rb, so any (ka, kb) serves there
+      if constexpr (ROLE != 0) {
+        // entries not loaded are zero in ra and rb, so any coefficients serve there
         const int e = idx / W;
         const Slot sl = HOIST ? ts : slot(idx % W);
         double fc;
         int src = pad_src(e, a.rows_in, L, fc);
+        if (src < 0) src = 0;
         int ia, ib;
         T ka, kb;
-        curl_k(cfg, src < 0 ? 0 : src, sl.j, sl.l, ia, ib, ka, kb);
+        if constexpr (CURL) vort_k(src, sl.l, ka, kb);
+        else curl_k(cfg, src, sl.j, sl.l, ia, ib, ka, kb);
         v = mul_i<T>(mk<T>(ka * ra[q].x - kb * rb[q].x, ka * ra[q].y - kb * rb[q].y));
       }
       if constexpr (INV) {
@@ -1361,16 +1385,16 @@ __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __rest
   // last pass stores on its own).
   constexpr RadixPlan plD = make_plan(L, RP);
   constexpr bool DIRECT =
-      PIPE == 1 && (KIND == K_FWD || ((KIND == K_INV || UC) && plD.n >= 2));
+      PIPE == 1 && (KIND == K_FWD || ((KIND == K_INV || UC || CURL) && plD.n >= 2));
   if constexpr (!DIRECT) {
     for (int base = 0; base < NIT; base += CH) {
       cplx<T> ra[CH], rb[CH];
-      if (UC && curlb) {
-        issue(base, ra, rb, BoolC<true>{});
-        commit(base, ra, rb, BoolC<true>{});
+      if ((UC || CURL) && curlb) {
+        issue(base, ra, rb, IntC<1>{});
+        commit(base, ra, rb, IntC<1>{});
       } else {
-        issue(base, ra, rb, BoolC<false>{});
-        commit(base, ra, rb, BoolC<false>{});
+        issue(base, ra, rb, IntC<0>{});
+        commit(base, ra, rb, IntC<0>{});
       }
     }
     __syncthreads();
@@ -1387,7 +1411,7 @@ __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __rest
     if (nxt) {
       c0 += TC;
       if constexpr (HOIST) ts = slot(threadIdx.x % W);
-      issue(0, pa, pb, BoolC<false>{});
+      issue(0, pa, pb, IntC<0>{});
       c0 -= TC;
       if constexpr (HOIST) ts = slot(threadIdx.x % W);
     }
@@ -1441,6 +1465,24 @@ __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __rest
               }
             }
             v = curlb ? mul_i<T>(mk<T>(ka * ra.x - kb * rb.x, ka * ra.y - kb * rb.y)) : ra;
+          } else if constexpr (CURL) {
+            // as issue + commit: the 3/2 padding source row and the 2/3
+            // rule's zero rows, then one field or the vorticity of two
+            double fc;
+            int src = pad_src(e, a.rows_in, L, fc);
+            if (a.prune && src >= 0 && abs(cmode(src, a.rows_in)) > a.Krow) src = -1;
+            if (sl.ld && src >= 0) {
+              const cplx<T>* p = pcol + (long long)src * a.in_row;
+              if (!curlb) {
+                v = ld_stream<cp_load(CP), T>(p + (long long)cfg * a.in_field);
+              } else {
+                T ka, kb;
+                vort_k(src, sl.l, ka, kb);
+                const cplx<T> ra = ld_stream<cp_load(CP), T>(p + 2 * a.in_field);
+                const cplx<T> rb = ld_stream<cp_load(CP), T>(p + a.in_field);
+                v = mul_i<T>(mk<T>(ka * ra.x - kb * rb.x, ka * ra.y - kb * rb.y));
+              }
+            }
           } else if (sl.ld) {
             if constexpr (INV) {
               // the inverse's row map: 3/2 padding source row (or none),
@@ -1594,7 +1636,7 @@ __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __rest
       __syncthreads();  // every read of this tile's LDS is done
       c0 += TC;
       if constexpr (HOIST) ts = slot(threadIdx.x % W);
-      commit(0, pa, pb, BoolC<false>{});
+      commit(0, pa, pb, IntC<0>{});
       __syncthreads();
     }
   }

@@ -46,6 +46,7 @@ int col_pair_c(int kind, const ColArgs& a0, const void* tw0, const ColArgs& a1, 
     case K_FWD: return col_pair_k<T, L0, L1, K_FWD, CP>(a0, tw0, a1, tw1, s);
     case K_FWD_STAGE3: return col_pair_k<T, L0, L1, K_FWD_STAGE3, CP>(a0, tw0, a1, tw1, s);
     case K_INV_UC3: return col_pair_k<T, L0, L1, K_INV_UC3, CP>(a0, tw0, a1, tw1, s);
+    case K_INV_CURL3: return col_pair_k<T, L0, L1, K_INV_CURL3, CP>(a0, tw0, a1, tw1, s);
     default: return -4;
   }
 }

@@ -11,8 +11,8 @@ rows of axis 0, a1 = 2 floor(N1/3)+1 rows of axis 1 and a2 = floor(N2/3)+1
 entries of the last axis (3/2 rule: a0 = N0, a1 = N1, a2 = nh).
 
 3D                          read                          write
-  inv_axis0_curl            3 a0 a1 a2                    6 M0 a1 a2
-  inv_axis1                 6 M0 a1 a2                    6 M0 M1 a2
+  inv_axis0_curl            3 a0 a1 a2                    5 M0 a1 a2   (u, omega_y, omega_z)
+  inv_axis1                 5 M0 a1 a2                    6 M0 M1 a2
   row_c2r_cross_r2c         6 M0 M1 a2                    3 M0 M1 nh
   fwd_axis1                 3 M0 M1 nh                    3 M0 N1 nh
   fwd_axis0_stage (fused)   3 M0 N1 nh + u_hat, u0: 6 nk  u_hat: 3 nk
@@ -48,8 +48,8 @@ def pass_bytes(n, dealias="3/2", precision="fp32", batch=1, nonlinear=True, fuse
         a1 = 2 * (N1 // 3) + 1 if direct else N1
         a2 = N2 // 3 + 1 if direct else nh
         if nonlinear:
-            out["inv_axis0_curl"] = 3 * a0 * a1 * a2 + 6 * M0 * a1 * a2
-            out["inv_axis1"] = 6 * M0 * a1 * a2 + 6 * M0 * M1 * a2
+            out["inv_axis0_curl"] = 3 * a0 * a1 * a2 + 5 * M0 * a1 * a2
+            out["inv_axis1"] = 5 * M0 * a1 * a2 + 6 * M0 * M1 * a2
             out["row_c2r_cross_r2c"] = 6 * M0 * M1 * a2 + 3 * M0 * M1 * nh
             out["fwd_axis1"] = 3 * M0 * M1 * nh + 3 * M0 * N1 * nh
             if fused:

@@ -104,14 +104,15 @@ def test_perfmodel_matches_byte_table():
     nk = 256 * 256 * 129
     pb = perfmodel.pass_bytes([256] * 3, "2/3", "fp32", fused=False)
     assert pb["stage"] == 12 * nk * 8
-    # pruned inverse passes: 171 retained rows per complex axis, 86 entries on the last
-    assert pb["inv_axis1"] == (6 * 256 * 171 * 86 + 6 * 256 * 256 * 86) * 8
+    # pruned inverse passes: 171 retained rows per complex axis, 86 entries on the last;
+    # the axis-1 inverse reads 5 fields (u, v, w, omega_y, omega_z) and writes 6
+    assert pb["inv_axis1"] == (5 * 256 * 171 * 86 + 6 * 256 * 256 * 86) * 8
     fused = perfmodel.pass_bytes([256] * 3, "2/3", "fp32")
     assert fused["fwd_axis0_stage"] == (3 * 256 * 256 * 129 + 9 * nk) * 8
     assert "stage" not in fused
     # 3/2 rule pads the inverse outputs by 3/2 per padded axis, nothing pruned
     pb32 = perfmodel.pass_bytes([64] * 3, "3/2", "fp64")
-    assert pb32["inv_axis0_curl"] == (3 * 64 * 64 * 33 + 6 * 96 * 64 * 33) * 16
+    assert pb32["inv_axis0_curl"] == (3 * 64 * 64 * 33 + 5 * 96 * 64 * 33) * 16
 
 
 @pytest.mark.parametrize("kw,msg", [
