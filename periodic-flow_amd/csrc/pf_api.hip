@@ -1080,7 +1080,7 @@ int plan_geometry(pf_plan* p, const pf_desc& d, long long& w1c, long long& w2c) 
   if (d.nonlinear) {
     if (d.dim == 3) {
       const long long M0 = p->M[0], M1 = p->M[1], N0 = p->N[0], N1 = p->N[1];
-      const long long A = B * 6 * M0 * p->aplane, Bb = B * 6 * M0 * M1 * P, C = B * 3 * M0 * M1 * P;
+      const long long A = B * 5 * M0 * p->aplane, Bb = B * 6 * M0 * M1 * P, C = B * 3 * M0 * M1 * P;
       const long long D = B * 3 * M0 * p->dplane, R = B * 3 * N0 * N1 * nh;
       w1c = std::max(std::max(w1c, A), std::max(C, R));
       w2c = std::max(Bb, D);

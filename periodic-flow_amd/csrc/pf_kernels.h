@@ -1644,11 +1644,28 @@ __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __rest
 }
 
 
+// col_body with the cache policy CP, except that K_INV_CURL3's blocks of v
+// and w (fields 1, 2) and of omega_x (3, which re-reads v and w) load
+// through the caches when CP asks for nontemporal loads: only the blocks of
+// u, omega_y and omega_z (0, 4, 5), whose inputs no other block reads, load
+// nontemporally.  The field index is the fastest of the block order.
+template <typename T, int L, int TC, int KIND, int PIPE, int NT, int RP, int CP>
+__device__ __forceinline__ void col_run(const ColArgs& a, const cplx<T>* __restrict__ tw, cplx<T>* tws,
+                                        cplx<T>* buf, long long bid, long long nblk) {
+  if constexpr (KIND == K_INV_CURL3 && cp_load(CP)) {
+    const int fg = (int)(xcd_order(bid, nblk) % a.n_fields);
+    if (fg == 0 || fg >= 4) col_body<T, L, TC, KIND, PIPE, NT, RP, CP>(a, tw, tws, buf, bid, nblk);
+    else col_body<T, L, TC, KIND, PIPE, NT, RP, CP & 1>(a, tw, tws, buf, bid, nblk);
+  } else {
+    col_body<T, L, TC, KIND, PIPE, NT, RP, CP>(a, tw, tws, buf, bid, nblk);
+  }
+}
+
 template <typename T, int L, int TC, int KIND, int PIPE, int NT, int RP = 0, int CP = 0>
 __global__ void __launch_bounds__(NT) col_pass(ColArgs a, const cplx<T>* __restrict__ tw) {
   __shared__ cplx<T> tws[col_lds_tw<T, L, TC, KIND, RP>()];
   __shared__ cplx<T> buf[col_lds_buf<T, L, TC, KIND>()];
-  col_body<T, L, TC, KIND, PIPE, NT, RP, CP>(a, tw, tws, buf, blockIdx.x, gridDim.x);
+  col_run<T, L, TC, KIND, PIPE, NT, RP, CP>(a, tw, tws, buf, blockIdx.x, gridDim.x);
 }
 
 
@@ -1674,9 +1691,9 @@ __global__ void __launch_bounds__(kThreads) col_pass_pair(ColSeg s0, ColSeg s1, 
   const long long bid = blockIdx.x;
   if (bid < split) {
     if (bid < s0.nblk)
-      col_body<T, L0, TC0, KIND, 1, kThreads, 0, CP>(s0.a, (const cplx<T>*)s0.tw, tws, buf, bid, s0.nblk);
+      col_run<T, L0, TC0, KIND, 1, kThreads, 0, CP>(s0.a, (const cplx<T>*)s0.tw, tws, buf, bid, s0.nblk);
   } else {
-    col_body<T, L1, TC1, KIND, 1, kThreads, 0, CP>(s1.a, (const cplx<T>*)s1.tw, tws, buf, bid - split, s1.nblk);
+    col_run<T, L1, TC1, KIND, 1, kThreads, 0, CP>(s1.a, (const cplx<T>*)s1.tw, tws, buf, bid - split, s1.nblk);
   }
 }
 

@@ -120,7 +120,7 @@ class GpuBackend:
         M = [n if spec.dealias in ("2/3", "truncate") else 3 * n // 2 for n in spec.n]
         if spec.nonlinear:
             if d == 3:
-                w = (6 * M[0] * spec.n[1] * nh + 6 * M[0] * M[1] * nh) * c
+                w = (5 * M[0] * spec.n[1] * nh + 6 * M[0] * M[1] * nh) * c
             else:
                 w = (3 * M[0] * nh + 2 * M[0] * nh) * c
         else:
