@@ -273,14 +273,46 @@ template <int S, typename T> __device__ __forceinline__ cplx<T> twiddle(const cp
   return w;
 }
 
+// Whether every transform of a contiguous-row FFT (consecutive threads on
+// consecutive butterflies of one transform) stays within one wavefront from
+// pass to pass: every pass of the plan has the same butterfly count L / R,
+// and it divides the wavefront, so transform f is always handled by the
+// same aligned group of L / R lanes.  Its passes then exchange data through
+// LDS with wavefront-level synchronisation instead of block barriers.
+constexpr bool wave_local_plan(int L, int rp = 0) {
+  const RadixPlan pl = make_plan(L, rp);
+  for (int q = 0; q < pl.n; ++q)
+    if (pl.r[q] != pl.r[0]) return false;
+  const int nb = L / pl.r[0];
+  return 64 % nb == 0;
+}
+
+// The synchronisation between the read and write halves of a pass and
+// between passes: a block barrier, or (WL) a wavefront barrier with
+// release / acquire fences at wavefront scope, which orders one wave's LDS
+// accesses across lanes.
+template <bool WL> __device__ __forceinline__ void fft_sync() {
+  if constexpr (WL) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 // One Stockham pass and the passes after it (compile-time recursion).
 // Addr(f, e) gives the LDS index of element e of transform f.  FFAST maps
 // consecutive threads to consecutive transforms (column tiles); otherwise
 // to consecutive butterflies of one transform (contiguous rows).
 // PEND: stop before pass PEND (the caller runs the remaining passes).
-// RP: the radix plan (make_plan).
-template <typename T, int L, int S, int NF, int NT, int P, int NS, bool FFAST, class Addr, int PEND = 99, int RP = 0>
+// RP: the radix plan (make_plan).  WL: wavefront-level synchronisation
+// inside the transform (contiguous rows of a wave_local_plan only); the
+// last pass still ends with a block barrier.
+template <typename T, int L, int S, int NF, int NT, int P, int NS, bool FFAST, class Addr, int PEND = 99, int RP = 0,
+          bool WL = false>
 __device__ __forceinline__ void stockham(cplx<T>* buf, const cplx<T>* tw, const Addr& addr) {
+  static_assert(!WL || (!FFAST && wave_local_plan(L, RP) && NT % 64 == 0), "transforms must stay within a wave");
   constexpr RadixPlan pl = make_plan(L, RP);
   if constexpr (P < pl.n && P < PEND) {
     constexpr int R = pl.r[P];
@@ -305,7 +337,7 @@ __device__ __forceinline__ void stockham(cplx<T>* buf, const cplx<T>* tw, const 
         dft<R, S, T>(v[kk]);
       }
     }
-    __syncthreads();
+    fft_sync<WL>();
 #pragma unroll
     for (int kk = 0; kk < K; ++kk) {
       const int q = threadIdx.x + kk * NT;
@@ -318,17 +350,18 @@ __device__ __forceinline__ void stockham(cplx<T>* buf, const cplx<T>* tw, const 
         for (int r = 0; r < R; ++r) buf[addr(f, base + r * NS)] = v[kk][r];
       }
     }
-    __syncthreads();
-    stockham<T, L, S, NF, NT, P + 1, NS * R, FFAST, Addr, PEND, RP>(buf, tw, addr);
+    if constexpr (P + 1 < pl.n && P + 1 < PEND) fft_sync<WL>();
+    else __syncthreads();
+    stockham<T, L, S, NF, NT, P + 1, NS * R, FFAST, Addr, PEND, RP, WL>(buf, tw, addr);
   }
 }
 
 // NF independent length-L transforms held in LDS.  Callers synchronise the
 // block before the call; the call ends with a block barrier.
-template <typename T, int L, int S, int NF, int NT, bool FFAST, int RP = 0, class Addr>
+template <typename T, int L, int S, int NF, int NT, bool FFAST, int RP = 0, bool WL = false, class Addr>
 __device__ __forceinline__ void lds_fft(cplx<T>* buf, const cplx<T>* tw, const Addr& addr) {
   static_assert(plan_ok(L), "FFT length must be 2^a 3^b 5^c with 1 <= a <= 11");
-  stockham<T, L, S, NF, NT, 0, 1, FFAST, Addr, 99, RP>(buf, tw, addr);
+  stockham<T, L, S, NF, NT, 0, 1, FFAST, Addr, 99, RP, WL>(buf, tw, addr);
 }
 
 // All passes but the last (the caller runs it); ends with a block barrier

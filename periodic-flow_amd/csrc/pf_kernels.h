@@ -123,6 +123,9 @@ __device__ __forceinline__ void row_body(const RowArgs& a, const cplx<T>* __rest
   else b = by;
   constexpr int PD = row_pad_div(L);
   static_assert(L - 1 + (L - 1) / PD < LP, "row pitch too small for the padding");
+  // each row transform within one wavefront for the whole FFT: its passes
+  // synchronise the wave only
+  constexpr bool WL = wave_local_plan(L, RP);
   auto ix = [](int slot, int e) { return slot * LP + e + e / PD; };
   const int N = a.N;
 
@@ -219,9 +222,9 @@ __device__ __forceinline__ void row_body(const RowArgs& a, const cplx<T>* __rest
   __syncthreads();
 
   if constexpr (MODE == ROW_R2C) {
-    lds_fft<T, L, -1, NP, kThreads, false, RP>(buf, tws, ix);
+    lds_fft<T, L, -1, NP, kThreads, false, RP, WL>(buf, tws, ix);
   } else {
-    lds_fft<T, L, +1, NP * NS, kThreads, false, RP>(buf, tws, ix);
+    lds_fft<T, L, +1, NP * NS, kThreads, false, RP, WL>(buf, tws, ix);
   }
 
   if constexpr (MODE == ROW_C2R) {
@@ -264,7 +267,7 @@ __device__ __forceinline__ void row_body(const RowArgs& a, const cplx<T>* __rest
       const int slot = p * 6 + (w == 0 ? 0 : (w == 1 ? 3 : 1));
       return slot * LP + e + e / PD;
     };
-    lds_fft<T, L, -1, NP * 3, kThreads, false, RP>(buf, tws, fx);
+    lds_fft<T, L, -1, NP * 3, kThreads, false, RP, WL>(buf, tws, fx);
   } else if constexpr (MODE == ROW_FUSED2) {
     for (int t = threadIdx.x; t < NP * L; t += kThreads) {
       const int p = t / L, x = t % L;
@@ -280,7 +283,7 @@ __device__ __forceinline__ void row_body(const RowArgs& a, const cplx<T>* __rest
       const int p = ff / 2, w = ff % 2;
       return (p * 3 + 2 * w) * LP + e + e / PD;
     };
-    lds_fft<T, L, -1, NP * 2, kThreads, false, RP>(buf, tws, fx);
+    lds_fft<T, L, -1, NP * 2, kThreads, false, RP, WL>(buf, tws, fx);
   }
 
   // ---- split the packed spectra, truncate, scale, store.  TPO threads own
