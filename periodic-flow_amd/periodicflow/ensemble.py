@@ -114,7 +114,7 @@ class Ensemble:
         """Choose every group's launch plans by timing (Plan.autotune), then,
         with several groups, paired launches against the groups stepped one
         after another (``self.pair``: kept unless stepping apart is at least
-        3% faster), timed on the states themselves, which are then put back
+        1% faster), timed on the states themselves, which are then put back
         with every member's RNG step counter.  Returns {group key: {pass
         name: chosen ms}}.  A member then equals the same member run alone
         bit for bit only under the same plan (the radix factorisation changes
@@ -136,7 +136,7 @@ class Ensemble:
                 torch.cuda.synchronize()
                 dt = time.perf_counter() - t0
                 t[mode] = min(t.get(mode, dt), dt)
-            self.pair = not t[False] < 0.97 * t[True]
+            self.pair = not t[False] < 0.99 * t[True]
             for g, (st, host, msteps) in zip(gs, saved):
                 g.state.copy_(st)
                 g.plan.step_counter = host

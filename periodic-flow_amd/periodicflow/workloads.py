@@ -290,10 +290,12 @@ def main(argv=None):
     python -m periodicflow.workloads stokes512 --steps 600
         thermal-noise Stokes 512^2 x 256 realisations, register-resident
         steps with the FP64 structure factor accumulated
-    python -m periodicflow.workloads forced --steps 5 [--plan PLAN]
+    python -m periodicflow.workloads forced --steps 5 [--plan PLAN] [--separate]
         the forced-turbulence ensemble 16 x 128^3 + 16 x 96^3, default plans
         or the given ones (no autotuning, so the trace holds only the steps);
-        PLAN: 'N:pass=variant,...,fused=0|1;N:...' per group of grid size N
+        PLAN: 'N:pass=variant,...,fused=0|1;N:...' per group of grid size N;
+        --separate steps the two groups one after the other instead of with
+        paired launches
     python -m periodicflow.workloads tgv256 --steps 5
         Taylor-Green 256^3, RK4, 2/3 rule, FP32
 
@@ -306,6 +308,7 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--plan", default=None, help="forced: launch plans per group, 'N:pass=v,...,fused=0|1;N:...'")
+    ap.add_argument("--separate", action="store_true", help="forced: step the groups one after the other")
     args = ap.parse_args(argv)
     if args.workload == "stokes512":
         n, nu, dt, kT, B = 512, 1.0, 0.05, 1e-3, 256
@@ -332,6 +335,7 @@ def main(argv=None):
                     g.plan.set_tiles({k: int(v) for k, v in kv.items()})
                     if fused is not None:
                         g.plan.set_fusion(bool(int(fused)))
+        ens.pair = not args.separate
         t = _timed(lambda: ens.step(args.steps))
         rate = ens.points_per_step() * args.steps / t / 1e6
     else:
