@@ -545,8 +545,11 @@ __device__ __forceinline__ void stage_terms(const StageArgs& a, long long idx, i
     nz[i] = mk<T>(0, 0);
     fr[i] = mk<T>(0, 0);
   }
-  // nonlinear term: Leray projection (math.py:72-100) and optional Nyquist mask
-  if (!RES && (nlin || a.rhs_nl)) {
+  // The loads of the nonlinear term and of the correlation function are
+  // issued first and used after the noise draws, so the draws' arithmetic
+  // runs while the loads are in flight.
+  const bool has_nl = !RES && (nlin || a.rhs_nl);
+  if (has_nl) {
     if (nlin) {
 #pragma unroll
       for (int i = 0; i < D; ++i) nl[i] = nlin[i];
@@ -555,18 +558,15 @@ __device__ __forceinline__ void stage_terms(const StageArgs& a, long long idx, i
 #pragma unroll
       for (int i = 0; i < D; ++i) nl[i] = r[i * a.nk];
     }
-    project<T, D>(nl, m);
-    if (a.mask_nyq && m.nyq) {
-#pragma unroll
-      for (int i = 0; i < D; ++i) nl[i] = mk<T>(0, 0);
-    }
   }
+  const bool thermal = RES ? RES == 1 : a.noise_type == 0;
+  cplx<T> C = mk<T>(0, 0);
+  if (a.noise_on && !thermal) C = RES ? *cpre : ((const cplx<T>*)a.corr)[idx];
 
   // noise (space_solver.py:227-265)
   if (a.noise_on) {
     const T fac = m.k2 == (T)0 ? (T)0 : (T)a.noise_norm;
     const T na = (T)a.noise_a;
-    const bool thermal = RES ? RES == 1 : a.noise_type == 0;
     const unsigned long long seed = !RES && a.use_rng ? a.seeds[b] : 0ull;
     const int nc = thermal ? D * D : D;
     const cplx<T>* raw = !RES && a.noise ? (const cplx<T>*)a.noise + (long long)b * 2 * nc * a.nk + idx : nullptr;
@@ -632,7 +632,6 @@ __device__ __forceinline__ void stage_terms(const StageArgs& a, long long idx, i
           nz[i] = mul_i<T>(acc);
         }
       } else {
-        const cplx<T> C = RES ? *cpre : ((const cplx<T>*)a.corr)[idx];
 #pragma unroll
         for (int i = 0; i < D; ++i) nz[i] = mul<T>(C, w[i]);
       }
@@ -641,6 +640,15 @@ __device__ __forceinline__ void stage_terms(const StageArgs& a, long long idx, i
     const T mag = (T)a.noise_mag;
 #pragma unroll
     for (int i = 0; i < D; ++i) nz[i] = scl<T>(nz[i], mag);
+  }
+
+  // nonlinear term: Leray projection (math.py:72-100) and optional Nyquist mask
+  if (has_nl) {
+    project<T, D>(nl, m);
+    if (a.mask_nyq && m.nyq) {
+#pragma unroll
+      for (int i = 0; i < D; ++i) nl[i] = mk<T>(0, 0);
+    }
   }
 
   // constant-power band forcing
