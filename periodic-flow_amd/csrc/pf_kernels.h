@@ -1218,10 +1218,10 @@ __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __rest
   const ColGeom& gm = a.g;
   // PIPE consecutive tiles of one (field, matrix, member) per block; the
   // tile index varies fastest, so the blocks an XCD runs together cover
-  // neighbouring columns of the same rows; the velocity + vorticity kinds
-  // take the field fastest instead, so that the blocks re-reading a tile's
-  // u_hat components for the curl run beside the one that loaded them and
-  // find them in the same XCD's L2
+  // neighbouring columns of the same rows; the kinds that form curl
+  // components (K_INV_UC3 / UC2, K_INV_CURL3) take the field fastest
+  // instead, so that the blocks re-reading a tile's components for the curl
+  // run beside the ones that loaded them and find them in the same XCD's L2
   const int ntb = (gm.n_tiles + PIPE - 1) / PIPE;
   int tb, fg;
   if constexpr (UC || CURL) {
