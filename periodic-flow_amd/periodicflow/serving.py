@@ -82,7 +82,7 @@ class GpuBackend:
     """Pools as batched HIP plans; states are device tensors.
 
     Pools are dealt round-robin to ``n_streams`` HIP streams (default
-    priority; the box exposes 4 hardware queues per process), so the steps
+    priority; HIP gives a process 4 hardware queues by default), so the steps
     of different pools overlap on the device.  Loads, moves and read-backs
     run on the default stream; each round waits for them before stepping and
     the engine synchronises the device at the end of every round."""
