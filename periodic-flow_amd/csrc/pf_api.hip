@@ -954,6 +954,51 @@ template <typename T> int tune_round(pf_plan* p, void* u, bool fused_ok, int sta
   return 0;
 }
 
+// One stage of the plan's own launch sequence on u (front passes, then the
+// fused axis-0 forward + stage pass or the unfused pair), for timing the
+// sequence as a whole.
+template <typename T> int sequence_round(pf_plan* p, void* u, hipStream_t s) {
+  const int ns = n_stages(p->d.integrator);
+  const bool noise_on = p->d.noise_mag != 0.0 && uses_noise(p->d.integrator);
+  StageArgs a = stage_args(p, u, ns > 1 ? 1 : 0, nullptr, noise_on);
+  a.save_u0 = 0;
+  PF_TRY(nonlinear_front<T>(p, u, s));
+  if (p->fuse_stage) {
+    StageArgs f = a;
+    f.rhs_nl = nullptr;
+    PF_TRY(last_forward<T>(p, p->dim == 3 ? K_FWD_STAGE3 : K_FWD_STAGE2, nullptr, &f, s));
+  } else {
+    PF_TRY(last_forward<T>(p, K_FWD, p->w1, nullptr, s));
+    PF_TRY(launch_stage<T>(p->dim, a, s, stage_cache_policy(p)));
+  }
+  return 0;
+}
+
+// Milliseconds per stage of n back-to-back stages of the current plan (the
+// lower of two runs), or a negative value on failure.
+template <typename T> double sequence_ms(pf_plan* p, void* u, int n, hipStream_t s) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess) return -1.0;
+  if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return -1.0; }
+  double best = -1.0;
+  for (int rep = 0; rep < 2; ++rep) {
+    if (hipEventRecord(e0, s) != hipSuccess) break;
+    int rc = 0;
+    for (int i = 0; i < n && !rc; ++i) rc = sequence_round<T>(p, u, s);
+    float ms = 0.f;
+    if (rc || hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+      best = -1.0;
+      break;
+    }
+    const double t = ms / (double)n;
+    if (best < 0.0 || t < best) best = t;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return best;
+}
+
 template <typename T> int autotune_impl(pf_plan* p, const void* uhat, int reps, double* ms_out, hipStream_t s) {
   if (ms_out)
     for (int i = 0; i < PF_NPASS; ++i) ms_out[i] = 0.0;
@@ -967,10 +1012,15 @@ template <typename T> int autotune_impl(pf_plan* p, const void* uhat, int reps, 
   const uint32_t mask0 = p->prof_mask;
   p->prof_mask = 0xffffffffu;
   const int tunable[] = {PF_PASS_INV0, PF_PASS_INV1, PF_PASS_ROW, PF_PASS_FWD1, PF_PASS_FWD0, PF_PASS_FWD0_STAGE};
-  double best[PF_NPASS], def[PF_NPASS];
+  double best[PF_NPASS], def[PF_NPASS], tv[PF_NPASS][kVariants];
   int bestv[PF_NPASS];
-  for (int i = 0; i < PF_NPASS; ++i) { best[i] = def[i] = -1.0; bestv[i] = 0; }
+  for (int i = 0; i < PF_NPASS; ++i) {
+    best[i] = def[i] = -1.0;
+    bestv[i] = 0;
+    for (int v = 0; v < kVariants; ++v) tv[i][v] = -1.0;
+  }
   double stage_ms = -1.0, stage_def = -1.0;
+  double stage_by_cp[2] = {-1.0, -1.0};  // the stage kernel's best time, ordinary / nontemporal
   int stage_cp = 0;
   int rc = 0;
   for (int v = -1; v < kVariants && !rc; ++v) {  // v = -1: warm-up round at the defaults
@@ -986,12 +1036,15 @@ template <typename T> int autotune_impl(pf_plan* p, const void* uhat, int reps, 
     for (int id : tunable) {
       if (!variant_ok<T>(p, id, v)) continue;
       const double t = avg_ms(p, id);
+      tv[id][v] = t;
       if (v == 0) def[id] = t;
       if (t >= 0.0 && (best[id] < 0.0 || t < best[id])) { best[id] = t; bestv[id] = v; }
     }
     const double ts = avg_ms(p, PF_PASS_STAGE);
     if (ts >= 0.0 && v == 0) stage_def = ts;
     if (ts >= 0.0 && (stage_ms < 0.0 || ts < stage_ms)) { stage_ms = ts; stage_cp = scp; }
+    double& sb = stage_by_cp[scp ? 1 : 0];
+    if (ts >= 0.0 && (sb < 0.0 || ts < sb)) sb = ts;
   }
   pf_profile_enable(p, prof0);
   p->prof_mask = mask0;
@@ -1023,9 +1076,63 @@ template <typename T> int autotune_impl(pf_plan* p, const void* uhat, int reps, 
   // traffic varies by several percent, so a tie goes to the unfused pair.
   if (best[PF_PASS_FWD0_STAGE] >= 0.0 && best[PF_PASS_FWD0] >= 0.0 && stage_ms >= 0.0)
     p->fuse_stage = best[PF_PASS_FWD0_STAGE] < kMargin * (best[PF_PASS_FWD0] + stage_ms);
+
+  // Refinement on the whole sequence.  The passes' choices interact through
+  // the caches: a pass's stores decide what the next pass finds in the
+  // 256 MB last-level cache (e.g. at 256^3 the axis-0 forward pass with
+  // ordinary stores leaves R there for the stage kernel), which timing each
+  // pass within rounds of one variant does not see.  Each other variant of
+  // each pass in use (within 15% of the chosen one on its own), the stage
+  // kernel's cache policy and the fusion choice are tried on consecutive
+  // stages of the plan's own sequence and kept where the sequence gets at
+  // least 1% faster.
+  {
+    constexpr int kSeq = 6;
+    constexpr double kSeqMargin = 0.99;
+    double base = sequence_ms<T>(p, tmp.ptr, kSeq, s);
+    bool changed = false;
+    auto trial = [&](auto apply, auto undo) {
+      if (base < 0.0) return;
+      apply();
+      const double t = sequence_ms<T>(p, tmp.ptr, kSeq, s);
+      if (t >= 0.0 && t < kSeqMargin * base) {
+        base = t;
+        changed = true;
+      } else {
+        undo();
+      }
+    };
+    // a second sweep over the passes where the first changed the plan (a
+    // pass's best choice can depend on a later pass's)
+    for (int sweep = 0; sweep < 2 && (sweep == 0 || changed); ++sweep) {
+      changed = false;
+      for (int id : tunable) {
+        const bool used = p->fuse_stage ? id != PF_PASS_FWD0 : id != PF_PASS_FWD0_STAGE;
+        if (!used) continue;
+        for (int v = 0; v < kVariants; ++v) {
+          const int cur = p->tsel[id];
+          if (v == cur || tv[id][v] < 0.0 || !variant_ok<T>(p, id, v)) continue;
+          // variants far slower on their own are not worth a trial
+          if (tv[id][cur] > 0.0 && tv[id][v] > 1.15 * tv[id][cur]) continue;
+          trial([&] { p->tsel[id] = v; }, [&] { p->tsel[id] = cur; });
+        }
+      }
+    }
+    if (!p->fuse_stage && !corrector_only(p)) {
+      const int cur = p->tsel[PF_PASS_STAGE];
+      trial([&] { p->tsel[PF_PASS_STAGE] = cur ? 0 : 7; }, [&] { p->tsel[PF_PASS_STAGE] = cur; });
+    }
+    if (tile_cols<T>(p->M[0], p->dim == 3 ? K_FWD_STAGE3 : K_FWD_STAGE2, p->tsel[PF_PASS_FWD0_STAGE]) > 0) {
+      const bool cur = p->fuse_stage;
+      trial([&] { p->fuse_stage = !cur; }, [&] { p->fuse_stage = cur; });
+    }
+    const hipError_t e2 = hipStreamSynchronize(s);
+    if (e2 != hipSuccess) return fail(PF_EHIP, std::string("autotune: ") + hipGetErrorString(e2));
+  }
   if (ms_out) {
-    for (int id : tunable) ms_out[id] = best[id] < 0.0 ? 0.0 : best[id];
-    ms_out[PF_PASS_STAGE] = stage_ms < 0.0 ? 0.0 : stage_ms;
+    for (int id : tunable) ms_out[id] = tv[id][p->tsel[id]] < 0.0 ? 0.0 : tv[id][p->tsel[id]];
+    const double st = stage_by_cp[p->tsel[PF_PASS_STAGE] == 7 ? 1 : 0];
+    ms_out[PF_PASS_STAGE] = st < 0.0 ? 0.0 : st;
   }
   return PF_OK;
 }
