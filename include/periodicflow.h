@@ -254,7 +254,11 @@ int pf_set_tiles(pf_plan* plan, const int32_t* tiles);
 int pf_get_tiles(const pf_plan* plan, int32_t* tiles, int32_t* fuse_stage);
 /* Times every variant of every pass of the nonlinear right-hand side and
  * the fused against the unfused axis-0 forward pass on a copy of u_hat
- * (left unchanged), `reps` evaluations each, and keeps the fastest; the
+ * (left unchanged), `reps` evaluations each, and keeps the fastest; then
+ * refines that plan on the whole launch sequence (other variants of each
+ * pass, the stage kernel's cache policy and the fusion choice, each kept
+ * where consecutive stages run at least 1% faster), since a pass's stores
+ * decide what the next pass finds in the last-level cache.  The
  * FFTW_MEASURE counterpart.  Enqueued on `stream` and synchronised;
  * ms_out[PF_NPASS] (nullable) receives the chosen passes' average times. */
 int pf_autotune(pf_plan* plan, const void* u_hat, int32_t reps, double* ms_out, void* stream);
