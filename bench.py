@@ -322,17 +322,37 @@ def tgv64_vs_oracle(torch):
     from oracle import pf_oracle as po
     from periodicflow import workloads as W
 
-    po.set_fft_workers(len(os.sched_getaffinity(0)))
+    cores = len(os.sched_getaffinity(0))
+    po.set_fft_workers(cores)
     r = W.tgv64_rk3(200)
+    # the GPU's 200 FP64 steps timed on their own (the run above reads the
+    # diagnostics back every step)
+    from periodicflow.plan import Plan
+
+    p = Plan([64] * 3, 0.01, precision="fp64", integrator="rk3", viscosity=1 / 400, dealias="3/2")
+    uh = W.tgv_state(p)
+    p.step(uh, 5)
+    uh = W.tgv_state(p)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    p.step(uh, 200)
+    torch.cuda.synchronize()
+    t_gpu = time.perf_counter() - t0
     o = po.OracleSolver([64] * 3, 0.01, time_integrator="rk3", viscosity=1 / 400, dealias="3/2")
     o.initialize(po.taylor_green_3d(o.g))
     E, eps = [po.energy(o.g, o.u_hat)], [po.dissipation(o.g, o.u_hat, 1 / 400)]
+    t_cpu = 0.0
     for _ in range(200):
+        t0 = time.perf_counter()
         o.solve(1, noise=[(0.0, 0.0)])
+        t_cpu += time.perf_counter() - t0
         E.append(po.energy(o.g, o.u_hat))
         eps.append(po.dissipation(o.g, o.u_hat, 1 / 400))
     E, eps = np.array(E), np.array(eps)
-    return {"E0": float(r["energy"][0]), "E200": float(r["energy"][-1]),
+    pts = 64**3 * 200
+    return {"gpu_fp64_Mpoint_steps_per_s": pts / t_gpu / 1e6, "gpu_seconds": t_gpu,
+            "cpu_oracle_Mpoint_steps_per_s": pts / t_cpu / 1e6, "cpu_oracle_seconds": t_cpu, "cpu_cores": cores,
+            "E0": float(r["energy"][0]), "E200": float(r["energy"][-1]),
             "eps0": float(r["dissipation"][0]), "eps200": float(r["dissipation"][-1]),
             "E_max_rel_err_vs_oracle": float(np.max(np.abs(r["energy"] - E) / E)),
             "eps_max_rel_err_vs_oracle": float(np.max(np.abs(r["dissipation"] - eps) / eps)),
