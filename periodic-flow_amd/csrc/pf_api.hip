@@ -1089,6 +1089,8 @@ template <typename T> int autotune_impl(pf_plan* p, const void* uhat, int reps, 
   {
     constexpr int kSeq = 6;
     constexpr double kSeqMargin = 0.99;
+    const bool prof1 = p->prof;
+    p->prof = false;  // no per-pass events from these rounds
     double base = sequence_ms<T>(p, tmp.ptr, kSeq, s);
     bool changed = false;
     auto trial = [&](auto apply, auto undo) {
@@ -1126,6 +1128,7 @@ template <typename T> int autotune_impl(pf_plan* p, const void* uhat, int reps, 
       const bool cur = p->fuse_stage;
       trial([&] { p->fuse_stage = !cur; }, [&] { p->fuse_stage = cur; });
     }
+    p->prof = prof1;
     const hipError_t e2 = hipStreamSynchronize(s);
     if (e2 != hipSuccess) return fail(PF_EHIP, std::string("autotune: ") + hipGetErrorString(e2));
   }
