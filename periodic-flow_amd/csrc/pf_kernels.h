@@ -1184,8 +1184,8 @@ __device__ __forceinline__ long long xcd_order(long long bid, long long G) {
   return x * q + (x < r ? x : r) + bid / 8;
 }
 
-// an int as a type, to select a lambda's code path at compile time
-template <int I> struct IntC { static constexpr int value = I; };
+// a bool as a type, to select a lambda's code path at compile time
+template <bool B> struct BoolC { static constexpr bool value = B; };
 
 template <int KIND> constexpr int col_fields() {
   return KIND == K_FWD_STAGE3 ? 3 : (KIND == K_FWD_STAGE2 ? 2 : 1);
@@ -1306,11 +1306,10 @@ __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __rest
   };
   // Loads of this thread's LDS slots base .. base + NQ - 1 of the tile at c0
   // (NQ = the register arrays' length), all issued before any is used.
-  // ROLE: 1 for a curl block (a compile-time copy of curlb, so that the
+  // CB: a curl block (a compile-time copy of curlb, so that the
   // unrolled loops hold no per-entry branch between their loads)
-  auto issue = [&](int base, auto& ra, auto& rb, auto rl) {
-    constexpr int ROLE = decltype(rl)::value;
-    constexpr bool CB = ROLE == 1;
+  auto issue = [&](int base, auto& ra, auto& rb, auto cb) {
+    constexpr bool CB = decltype(cb)::value;
     constexpr int NQ = (int)(sizeof(ra) / sizeof(ra[0]));
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
@@ -1357,15 +1356,15 @@ __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __rest
     }
   };
   // ... combined (curl, 3/2-rule Nyquist halving) and stored to LDS
-  auto commit = [&](int base, const auto& ra, const auto& rb, auto rl) {
-    constexpr int ROLE = decltype(rl)::value;
+  auto commit = [&](int base, const auto& ra, const auto& rb, auto cb) {
+    constexpr bool CB = decltype(cb)::value;
     constexpr int NQ = (int)(sizeof(ra) / sizeof(ra[0]));
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const int idx = threadIdx.x + (base + q) * NT;
       if (base + q >= NIT || idx >= L * W) continue;
       cplx<T> v = ra[q];
-      if constexpr (ROLE != 0) {
+      if constexpr (CB) {
         // entries not loaded are zero in ra and rb, so any coefficients serve there
         const int e = idx / W;
         const Slot sl = HOIST ? ts : slot(idx % W);
@@ -1401,11 +1400,11 @@ __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __rest
     for (int base = 0; base < NIT; base += CH) {
       cplx<T> ra[CH], rb[CH];
       if ((UC || CURL) && curlb) {
-        issue(base, ra, rb, IntC<1>{});
-        commit(base, ra, rb, IntC<1>{});
+        issue(base, ra, rb, BoolC<true>{});
+        commit(base, ra, rb, BoolC<true>{});
       } else {
-        issue(base, ra, rb, IntC<0>{});
-        commit(base, ra, rb, IntC<0>{});
+        issue(base, ra, rb, BoolC<false>{});
+        commit(base, ra, rb, BoolC<false>{});
       }
     }
     __syncthreads();
@@ -1422,7 +1421,7 @@ __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __rest
     if (nxt) {
       c0 += TC;
       if constexpr (HOIST) ts = slot(threadIdx.x % W);
-      issue(0, pa, pb, IntC<0>{});
+      issue(0, pa, pb, BoolC<false>{});
       c0 -= TC;
       if constexpr (HOIST) ts = slot(threadIdx.x % W);
     }
@@ -1647,7 +1646,7 @@ __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __rest
       __syncthreads();  // every read of this tile's LDS is done
       c0 += TC;
       if constexpr (HOIST) ts = slot(threadIdx.x % W);
-      commit(0, pa, pb, IntC<0>{});
+      commit(0, pa, pb, BoolC<false>{});
       __syncthreads();
     }
   }
