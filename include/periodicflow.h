@@ -257,7 +257,7 @@ int pf_get_tiles(const pf_plan* plan, int32_t* tiles, int32_t* fuse_stage);
  * (left unchanged), `reps` evaluations each, and keeps the fastest; then
  * refines that plan on the whole launch sequence (other variants of each
  * pass, the stage kernel's cache policy and the fusion choice, each kept
- * where consecutive stages run at least 1% faster), since a pass's stores
+ * where consecutive stages run at least 0.5% faster), since a pass's stores
  * decide what the next pass finds in the last-level cache.  The
  * FFTW_MEASURE counterpart.  Enqueued on `stream` and synchronised;
  * ms_out[PF_NPASS] (nullable) receives the chosen passes' average times. */
