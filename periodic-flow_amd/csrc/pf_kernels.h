@@ -50,15 +50,16 @@ template <typename T, int MODE> constexpr int row_pairs(int L) {
 __host__ __device__ constexpr int row_pitch(int L) { return L + L / 16; }
 // Row entry e of a slot sits at e + e / row_pad_div(L): one skipped entry
 // every 16 for power-of-two lengths (the radix-16/8 strides then fall on
-// distinct banks), every 48 (96 at length 144) where L has a factor 3, whose
-// radix-9/12/24 first pass writes with stride 9, 12 or 24, and every 80
-// where L has a factor 5 (radix-5 first pass).  Chosen by
-// counting bank conflicts of every LDS access of the row pass (64 lanes in
-// groups of 16 over 32 banks and of 32 over 64); at lengths 96, 192, 288 and
-// 384 this halves them or better, e.g. 2.4 to 0.7 conflict cycles per
-// access at 192.
+// distinct banks), every 48 where L has a factor 3, whose radix-9/12/24
+// first pass writes with stride 9, 12 or 24, none at length 144, and every
+// 80 where L has a factor 5 (radix-5 first pass).  Chosen by counting bank
+// conflicts of every LDS access of the row pass (64 lanes in groups of 16
+// over 32 banks and of 32 over 64) and by timing; at lengths 96, 192, 288
+// and 384 this halves them or better, e.g. 2.4 to 0.7 conflict cycles per
+// access at 192.  At 144 (plan 9 x 16) no padding measured 2-4% faster than
+// one skipped entry every 96 or 32.
 __host__ __device__ constexpr int row_pad_div(int L) {
-  return L % 5 == 0 ? 80 : (L % 3 ? 16 : (L == 144 ? 96 : 48));
+  return L % 5 == 0 ? 80 : (L % 3 ? 16 : (L == 144 ? L : 48));
 }
 
 // Hermitian extension of a stored half spectrum to index k of a length-L
