@@ -10,8 +10,8 @@ each: nonlinear term through five FFT passes, then the fused stage update).
 Prints one JSON line.  ``value`` = 256^3 * steps / seconds / 1e6 with the
 state resident in HBM.  ``roofline`` prices the most expensive pass against
 the 8 TB/s HBM peak using the algorithmic bytes of periodicflow.perfmodel
-and the pass's mean duration from HIP events recorded around each of its
-launches inside the timed region.  ``cpu_baseline`` times the FP64 NumPy
+and the pass's mean duration from HIP events that each of its launches
+records at the kernel's start and end inside the timed region.  ``cpu_baseline`` times the FP64 NumPy
 oracle (scipy.fft over all host cores) on a bounded sample of the same
 workload, and ``accuracy_vs_oracle`` compares the GPU's FP64 and FP32 states
 with the oracle's after those steps from the same start.  The FP32 result is

@@ -95,25 +95,46 @@ struct pf_plan {
   struct LaunchLog* rec = nullptr;  // while set, pass launches are recorded instead of issued
 };
 
+pf::PendingEvents& pf::pending_events() {
+  static thread_local PendingEvents pe;
+  return pe;
+}
+
 namespace {
 
-// Brackets one launch with a pair of events on its stream when profiling.
+// Times one pass with a pair of events when profiling.  A pass that is one
+// kernel launch (kernel = true) hands the events to that launch, which
+// records them at the kernel's start and end; otherwise (or when the pass
+// launched nothing, e.g. while recording) they are recorded on the stream
+// around the pass.
 struct PassTimer {
   pf_plan* p;
   int id;
   hipStream_t s;
+  bool kernel;
   hipEvent_t a = nullptr, b = nullptr;
-  PassTimer(pf_plan* p_, int id_, hipStream_t s_) : p(p_), id(id_), s(s_) {
+  PassTimer(pf_plan* p_, int id_, hipStream_t s_, bool kernel_ = true) : p(p_), id(id_), s(s_), kernel(kernel_) {
     if (p && p->prof && ((p->prof_mask >> id) & 1u) && hipEventCreate(&a) == hipSuccess &&
         hipEventCreate(&b) == hipSuccess) {
-      (void)hipEventRecord(a, s);
+      if (kernel) pending_events() = {a, b};
+      else (void)hipEventRecord(a, s);
     } else {
       a = b = nullptr;
     }
   }
   ~PassTimer() {
     if (a && b) {
-      (void)hipEventRecord(b, s);
+      bool by_kernel = false;
+      if (kernel) {
+        PendingEvents& pe = pending_events();
+        if (pe.start == a) {  // no launch took them
+          pe.start = pe.stop = nullptr;
+          (void)hipEventRecord(a, s);
+        } else {
+          by_kernel = true;
+        }
+      }
+      if (!by_kernel) (void)hipEventRecord(b, s);
       p->ev[id].emplace_back(a, b);
     }
   }
@@ -636,7 +657,7 @@ template <typename T> int band_energy(pf_plan* p, const void* uhat, hipStream_t 
   ba.kmax2 = p->d.forcing_kmax * p->d.forcing_kmax;
   ba.kmax = p->d.forcing_kmax;
   ba.power = p->d.forcing_power;
-  { PassTimer t(p, PF_PASS_BAND, s); PF_TRY(launch_band<T>(p->dim, ba, p->nact, s)); }
+  { PassTimer t(p, PF_PASS_BAND, s, false); PF_TRY(launch_band<T>(p->dim, ba, p->nact, s)); }
   return 0;
 }
 
@@ -705,7 +726,7 @@ template <typename T> int resident_impl(pf_plan* p, void* uhat, long long n_step
   a.single_draw = 1;
   for (long long done = 0; done < n_steps;) {
     const int k = (int)std::min<long long>(kResidentChunk, n_steps - done);
-    { PassTimer t(p, PF_PASS_STAGE, s); PF_TRY(launch_stokes_steps<T>(p->dim, a, k, acc, s)); }
+    { PassTimer t(p, PF_PASS_STAGE, s, false); PF_TRY(launch_stokes_steps<T>(p->dim, a, k, acc, s)); }
     PF_TRY(launch_member_step(p->msteps, p->nact, s, k));
     done += k;
     p->step += k;

@@ -1,5 +1,7 @@
 // Host-side launchers (one translation unit per kernel family and precision).
 #pragma once
+#include <hip/hip_ext.h>
+
 #include "pf_kernels.h"
 
 // Transform lengths compiled into the library: N and 3N/2 for N in
@@ -13,6 +15,24 @@
 namespace pf {
 
 bool length_supported(int L);
+
+// Events that the next pass kernel launch records itself: set by the pass
+// timer around a pass that is one kernel, taken by that launch, so that the
+// timing brackets exactly the kernel's execution on the device (events
+// recorded as separate stream commands add a few microseconds per launch).
+struct PendingEvents {
+  hipEvent_t start = nullptr, stop = nullptr;
+};
+PendingEvents& pending_events();
+
+// A pass kernel launch, recording the pending events if there are any.
+template <typename F, typename... Args>
+inline void launch_pass_kernel(F kernel, dim3 grid, dim3 block, hipStream_t s, Args... args) {
+  PendingEvents& pe = pending_events();
+  const hipEvent_t a = pe.start, b = pe.stop;
+  pe.start = pe.stop = nullptr;
+  hipExtLaunchKernelGGL(kernel, grid, block, 0, s, a, b, 0, args...);
+}
 
 // Launch variants: V = 0 is the default tile (column passes) or row-pair
 // count (row passes); FP32 also builds V = 1 (half), V = 2 (twice) and, for

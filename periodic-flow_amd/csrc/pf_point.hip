@@ -49,9 +49,9 @@ template <typename T, int CP> static void stage_launch(int dim, const StageArgs&
                 (unsigned)a.n_members);
   };
   const bool wide = a.nk * a.n_members >= (4ll << 20);
-  if (dim == 3) hipLaunchKernelGGL((stage_kernel<T, 3, 1, CP>), grid(1), dim3(kThreads), 0, s, a);
-  else if (wide) hipLaunchKernelGGL((stage_kernel<T, 2, 4, CP>), grid(4), dim3(kThreads), 0, s, a);
-  else hipLaunchKernelGGL((stage_kernel<T, 2, 1, CP>), grid(1), dim3(kThreads), 0, s, a);
+  if (dim == 3) launch_pass_kernel(stage_kernel<T, 3, 1, CP>, grid(1), dim3(kThreads), s, a);
+  else if (wide) launch_pass_kernel(stage_kernel<T, 2, 4, CP>, grid(4), dim3(kThreads), s, a);
+  else launch_pass_kernel(stage_kernel<T, 2, 1, CP>, grid(1), dim3(kThreads), s, a);
 }
 template <typename T> int launch_stage(int dim, const StageArgs& a, hipStream_t s, int cp) {
   if (cp == 3) stage_launch<T, 3>(dim, a, s);
