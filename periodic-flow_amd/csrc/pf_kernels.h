@@ -1209,7 +1209,10 @@ __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __rest
   constexpr int NFLD = KIND == K_FWD_STAGE3 ? 3 : (KIND == K_FWD_STAGE2 ? 2 : 1);
   constexpr int W = NFLD * TC;  // LDS columns
   constexpr int NIT = (L * W + NT - 1) / NT;
-  constexpr int CH = NIT < 16 ? NIT : 16;  // loads kept in flight per thread
+  // loads kept in flight per thread: 16, or 8 for the FP64 kinds that form
+  // curl components (two loads per entry, 4 registers each)
+  constexpr int CHM = (sizeof(T) == 8 && (UC || CURL)) ? 8 : 16;
+  constexpr int CH = NIT < CHM ? NIT : CHM;
   static_assert(NT % TC == 0, "tile width must divide the block size");
   constexpr int TWN = tw_size(L, RP);  // a constant, as in row_pass
   for (int m = threadIdx.x; m < TWN; m += NT) tws[m] = tw[m];
@@ -1393,10 +1396,13 @@ __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __rest
   // us for the axis-1 forward at 256^3).  The stage kinds keep it (there the
   // direct form measured 3-7% slower at lengths 192 and 144, whose first
   // radix is 3), and so does an inverse whose plan has a single pass (its
-  // last pass stores on its own).
+  // last pass stores on its own), and so do the FP64 kinds that form curl
+  // components: two loads of 16 bytes per entry of a radix-16 butterfly held
+  // 199 registers (2 waves per SIMD) at 256^3, and staged in chunks of 8 the
+  // axis-0 inverse went from 194 to 175 us.
   constexpr RadixPlan plD = make_plan(L, RP);
   constexpr bool DIRECT =
-      PIPE == 1 && (KIND == K_FWD || ((KIND == K_INV || UC || CURL) && plD.n >= 2));
+      PIPE == 1 && (KIND == K_FWD || ((KIND == K_INV || ((UC || CURL) && sizeof(T) == 4)) && plD.n >= 2));
   if constexpr (!DIRECT) {
     for (int base = 0; base < NIT; base += CH) {
       cplx<T> ra[CH], rb[CH];
