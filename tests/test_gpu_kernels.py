@@ -349,3 +349,37 @@ def test_set_tiles_rejects_missing_variants(pf):
     with pytest.raises(ValueError):
         plan.set_tiles({"no_such_pass": 1})
     plan.set_tiles({"inv_axis1": 0})
+
+
+def test_pass_profile_records_each_launch(pf):
+    """Per-pass timing (Plan.profile / profile_read): a single-kernel pass
+    hands its events to the launch, which records them at the kernel's start
+    and end; the band reduction of the forcing is timed by events around it.
+    Every pass of an unfused forced 3D step reports one record per launch
+    with a positive duration, and their sum stays below the wall time."""
+    import time
+
+    _say("pass profile 32^3 rk3 forced, unfused")
+    rs = np.random.RandomState(5)
+    plan = pf.Plan((32, 32, 32), 1e-3, precision="fp32", integrator="rk3", viscosity=0.01, dealias="3/2",
+                   noise_mag=0.05, forcing="constant_power", forcing_power=0.1, seeds=[7])
+    plan.set_fusion(False)
+    u = _dev(0.1 * _rand_spec(rs, (1, 3) + plan.shape_fourier), plan.complex_dtype)
+    plan.project(u)
+    plan.step(u, 1)
+    torch.cuda.synchronize()
+    plan.profile(True)
+    t0 = time.perf_counter()
+    plan.step(u, 2)
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - t0) * 1e3
+    rec = plan.profile_read()
+    plan.profile(False)
+    total = 0.0
+    for name in ("inv_axis0_curl", "inv_axis1", "row_c2r_cross_r2c", "fwd_axis1", "fwd_axis0", "stage", "band"):
+        ms, n = rec[name]
+        assert n == 6, (name, n)  # 2 steps x 3 stages
+        assert ms > 0.0, (name, ms)
+        total += ms
+    assert rec["fwd_axis0_stage"][1] == 0
+    assert total < wall_ms, (total, wall_ms)
