@@ -1397,12 +1397,14 @@ __device__ __forceinline__ void col_body(const ColArgs& a, const cplx<T>* __rest
   // direct form measured 3-7% slower at lengths 192 and 144, whose first
   // radix is 3), and so does an inverse whose plan has a single pass (its
   // last pass stores on its own), and so do the FP64 kinds that form curl
-  // components: two loads of 16 bytes per entry of a radix-16 butterfly held
-  // 199 registers (2 waves per SIMD) at 256^3, and staged in chunks of 8 the
-  // axis-0 inverse went from 194 to 175 us.
+  // components where the first radix is 16: two loads of 16 bytes per entry
+  // of a radix-16 butterfly held 199 registers (2 waves per SIMD) at 256^3,
+  // and staged in chunks of 8 the axis-0 inverse went from 194 to 175 us.
+  // With a smaller first radix (6 at length 96) the direct form stays faster
+  // (the FP64 64^3 step 5% faster than staged).
   constexpr RadixPlan plD = make_plan(L, RP);
   constexpr bool DIRECT =
-      PIPE == 1 && (KIND == K_FWD || ((KIND == K_INV || ((UC || CURL) && sizeof(T) == 4)) && plD.n >= 2));
+      PIPE == 1 && (KIND == K_FWD || ((KIND == K_INV || ((UC || CURL) && (sizeof(T) == 4 || plD.r[0] < 16))) && plD.n >= 2));
   if constexpr (!DIRECT) {
     for (int base = 0; base < NIT; base += CH) {
       cplx<T> ra[CH], rb[CH];
